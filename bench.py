@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched sFFT v2 hot path on one MI355X.
+
+Workload: 256 signals, each n = 2^22 complex double, k = 64 unit-magnitude
+random-phase tones plus complex Gaussian noise at SNR 40 dB (synthetic, built
+on the GPU).  One step = one sfft_exec_many_sparse over the whole batch with
+the inputs already resident in HBM.
+
+Reported beside the headline (FP32 bucketization + FP64 estimation, batched):
+  * the plain FP64 form: sfft_exec per signal into a dense, zero-filled
+    device output -- the reference's calling convention on the GPU;
+  * the FP64 batched form;
+  * p50/p99 latency of single-signal execs;
+  * accuracy: located-set diff and value diff FP32+refine vs FP64, and the
+    per-coefficient L1 error against an FP64 full FFT of the signal;
+  * roofline of the dominant kernel (fold + bucket FFT) from the library's
+    own HIP events on its stream;
+  * the CPU oracle timed on this host (OpenMP over signals) on a bounded
+    sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("sFFT transforms/s + p50/p99 latency, n=2^22 k=64: "
+          "FP32+refine vs FP64 GPU vs CPU")
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--snr", type=float, default=40.0)
+    ap.add_argument("--lat-samples", type=int, default=200)
+    ap.add_argument("--plain-signals", type=int, default=64)
+    ap.add_argument("--cpu-signals", type=int, default=256)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-fp32", action="store_true",
+                    help="time only the headline form (for profilers)")
+    return ap.parse_args()
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(batches shard by signal across GPUs with no exchange; see DESIGN.md)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    for v in ("RANK", "WORLD_SIZE"):
+        if int(os.environ.get(v, "0") or 0) > (0 if v == "RANK" else 1):
+            print("bench.py: multi-rank launch is not supported", file=sys.stderr)
+            sys.exit(2)
+    import torch
+    import sfft_amd
+    from sfft_amd.signals import signals_device
+
+    n, k, S = args.n, args.k, args.batch
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    seeds = list(range(1000, 1000 + S))
+    x, fs, vs = signals_device(n, k, seeds, snr_db=args.snr, device=dev)
+    xs = [x[i] for i in range(S)]
+    torch.cuda.synchronize()
+
+    # ---------------------------------------------------------- headline
+    p32 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp32_refine", seed=(17, 4242))
+    cap = p32.capacity
+    fr32, va32 = p32.make_sparse_outputs(S)
+    for _ in range(args.warmup):
+        p32.execute_many_sparse(xs, fr32, va32)
+    torch.cuda.synchronize()
+    p32.enable_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        counts32 = p32.execute_many_sparse(xs, fr32, va32)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages32 = p32.stage_times()
+    p32.enable_timing(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    tps32 = S * args.steps / (t1 - t0)
+
+    result = {
+        "metric": METRIC, "value": tps32, "unit": "transforms/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "sFFT v2 batched exec_many_sparse, FP32 buckets + FP64 "
+                               "estimation, %d signals, n=2^%d, k=%d, SNR %g dB"
+                               % (S, int(np.log2(n)), k, args.snr),
+                   "n": n, "k": k, "batch": S, "version": 2, "snr_db": args.snr,
+                   "params": dict(zip(["B_loc", "B_est", "B_thresh", "W_comb",
+                                       "comb_loops", "loc_loops", "est_loops",
+                                       "thr_loops", "w_loc", "w_est"], p32.params()))},
+    }
+
+    # ---------------------------------------------------------- roofline
+    prm = result["config"]["params"]
+    L = prm["loc_loops"] + prm["est_loops"]
+    w_sum = prm["loc_loops"] * prm["w_loc"] + prm["est_loops"] * prm["w_est"]
+    xs_size = prm["loc_loops"] * prm["B_loc"] + prm["est_loops"] * prm["B_est"]
+    # per launch (one batch): gathered samples 16 B each, FP32 taps once,
+    # FP32 bucket writes
+    fold_bytes = S * (16 * w_sum + 8 * xs_size) + 8 * (prm["w_loc"] + prm["w_est"])
+    fold_ms, fold_launches = stages32["fold_fft"]
+    fold_avg_ms = fold_ms / max(1, fold_launches)
+    achieved = fold_bytes / (fold_avg_ms * 1e-3) / 1e9
+    result["roofline"] = {"kernel": "k_fold_fft32", "bound": "hbm", "achieved": achieved,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": None, "alg_bytes_per_launch": fold_bytes,
+                          "avg_launch_ms": fold_avg_ms, "launches": fold_launches}
+    result["stages_ms_per_step"] = {s: v[0] / args.steps for s, v in stages32.items() if v[1]}
+
+    if not args.only_fp32:
+        # ------------------------------------------------------ latency
+        lat = []
+        one_f, one_v = fr32[:1], va32[:1]
+        for i in range(args.lat_samples):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            p32.execute_many_sparse([xs[i % S]], one_f, one_v)
+            lat.append((time.perf_counter() - ts) * 1e3)
+        result["latency_ms"] = {"p50": pct(lat, 50), "p99": pct(lat, 99),
+                                "samples": len(lat), "form": "fp32+refine, 1 signal"}
+
+        # ------------------------------------------------------ FP64 forms
+        p64 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp64", seed=(17, 4242))
+        fr64, va64 = p64.make_sparse_outputs(S)
+        p64.execute_many_sparse(xs, fr64, va64)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            counts64 = p64.execute_many_sparse(xs, fr64, va64)
+        torch.cuda.synchronize()
+        tps64b = S * max(1, args.steps // 2) / (time.perf_counter() - t0)
+
+        # same draws for both forms: replay from the same seed
+        p64.seed(17, 4242)
+        p32.seed(17, 4242)
+        counts64 = p64.execute_many_sparse(xs, fr64, va64)
+        counts32 = p32.execute_many_sparse(xs, fr32, va32)
+        torch.cuda.synchronize()
+        set_diff, vmax = 0, 0.0
+        for i in range(S):
+            c32, c64 = int(counts32[i]), int(counts64[i])
+            a, b = fr32[i, :c32], fr64[i, :c64]
+            if c32 != c64 or not torch.equal(a, b):
+                set_diff += 1
+                continue
+            vmax = max(vmax, float((va32[i, :c32] - va64[i, :c64]).abs().max()))
+
+        # plain FP64 form: one sfft_exec per signal into a dense device output
+        plain = sfft_amd.sfft(n, k, 2, strict=False, precision="fp64", seed=(17, 4242))
+        dense = torch.empty(n, dtype=torch.complex128, device=dev)
+        plain.execute_device(xs[0], dense)
+        torch.cuda.synchronize()
+        lat64 = []
+        P = min(args.plain_signals, S)
+        t0 = time.perf_counter()
+        for i in range(P):
+            ts = time.perf_counter()
+            plain.execute_device(xs[i], dense)
+            lat64.append((time.perf_counter() - ts) * 1e3)
+        torch.cuda.synchronize()
+        tps_plain = P / (time.perf_counter() - t0)
+        result["fp64_plain"] = {"value": tps_plain, "unit": "transforms/s",
+                                "p50_ms": pct(lat64, 50), "p99_ms": pct(lat64, 99),
+                                "signals": P, "form": "sfft_exec per signal, dense zero-filled "
+                                                      "device output (reference convention)"}
+        result["fp64_batched"] = {"value": tps64b, "unit": "transforms/s"}
+        result["speedup_fp32_vs_fp64_plain"] = tps32 / tps_plain
+        result["speedup_fp32_vs_fp64_batched"] = tps32 / tps64b
+
+        # ------------------------------------------------------ accuracy
+        # FP64 full FFT of the signal as the oracle; the reference's v1/v2
+        # values are conj(X), so compare with conj.
+        l1_loc, l1_true = [], []
+        for i in range(min(4, S)):
+            Xf = torch.fft.fft(xs[i]) / n
+            c = int(counts32[i])
+            f_idx = fr32[i, :c].long()
+            l1_loc.append(float((va32[i, :c] - Xf[f_idx].conj()).abs().mean()))
+            pos = {int(ff): j for j, ff in enumerate(fr32[i, :c].cpu().numpy())}
+            tv = [abs(complex(va32[i, pos[int(ff)]].item()) - np.conj(complex(Xf[int(ff)].item())))
+                  for ff in fs[i] if int(ff) in pos]
+            l1_true.append(float(np.mean(tv)) if tv else float("nan"))
+        result["accuracy"] = {
+            "located_set_diff_fp32_vs_fp64": set_diff,
+            "value_maxabs_fp32_vs_fp64": vmax,
+            "l1_vs_fp64_fft_located_set": float(np.mean(l1_loc)),
+            "l1_vs_fp64_fft_true_support": float(np.mean(l1_true)),
+            "located_per_signal": int(counts32[0]),
+        }
+
+    # ---------------------------------------------------------- CPU
+    if not args.no_cpu and not args.only_fp32:
+        result["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(args):
+    """The CPU oracle (restatement of the reference) on this host's cores."""
+    import ctypes as C
+    import oracle
+    from sfft_amd.signals import signal_np
+
+    n, k = args.n, args.k
+    distinct = min(16, args.cpu_signals)
+    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    P = oracle.V12(n, k, True)
+    xs = [signal_np(n, k, 1000 + i, snr_db=args.snr)[0] for i in range(distinct)]
+    outs = [np.empty(n, np.complex128) for _ in range(threads)]  # one per worker
+    num = args.cpu_signals
+    L = P.loops
+    oracle.seed_glibc(17, 4242)
+    a = np.zeros((num, L), np.int32)
+    ai = np.zeros((num, L), np.int32)
+    off = np.zeros((num, max(1, P.p.comb_loops)), np.int32)
+    for i in range(num):
+        a[i], ai[i], off[i] = P.draw()
+    xp = np.array([xs[i % distinct].ctypes.data for i in range(num)], dtype=np.uint64)
+    op = np.array([o.ctypes.data for o in outs], dtype=np.uint64)
+    t0 = time.perf_counter()
+    P.exec_many(xp, a, ai, off, op, -threads)
+    dt = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": num / dt, "unit": "transforms/s", "cores": threads, "kind": "port",
+            "sample": "%d execs (%d distinct signals) of v2 n=2^%d k=%d SNR %g dB; "
+                      "reference semantics incl. the dense zero-filled output; "
+                      "OpenMP over signals; %.1f s wall"
+                      % (num, distinct, int(np.log2(n)), k, args.snr, dt),
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * sfft_amd -- MI355X-native Sparse FFT behind the reference sFFT C API.
+ *
+ * The first block of entry points is the reference's public API
+ * (src/sfft.h:158-167 of davidediger/sfft) with the same names, argument
+ * meaning and enum values; a program written against the reference links
+ * against libsfft_amd.so unchanged.  The second block adds what the
+ * reference documents but does not export (a sparse result) and what a GPU
+ * caller needs (precision, seeded RNG streams, streams, errors).
+ *
+ * Pointers passed to sfft_exec / sfft_exec_many may be host or device
+ * (HIP) pointers; the library detects which.  Device-resident inputs avoid
+ * the PCIe copy of the whole signal.
+ */
+#ifndef SFFT_AMD_SFFT_H
+#define SFFT_AMD_SFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> complex_t;
+extern "C" {
+#else
+#include <complex.h>
+typedef double complex complex_t;
+#endif
+
+/* src/sfft.h:37-42 */
+typedef enum sfft_version {
+  SFFT_VERSION_1 = 0,
+  SFFT_VERSION_2 = 1,
+  SFFT_VERSION_3 = 2
+} sfft_version;
+
+/* src/sfft.h:44-50: same leading members; data is the library's plan. */
+typedef struct sfft_plan {
+  sfft_version version;
+  unsigned int n;
+  unsigned int k;
+  void *data;
+} sfft_plan;
+
+/* ---- reference API ------------------------------------------------------ */
+
+/* src/sfft.h:159 / src/sfft.cc:60-63: 16-byte aligned host allocation. */
+void *sfft_malloc(size_t s);
+/* src/sfft.h:160 / src/sfft.cc:65-69 */
+void sfft_free(void *p);
+
+/* src/sfft.h:162-163 / src/sfft.cc:71-101.  Builds parameters, filters and
+ * device-resident tap tables.  Returns NULL (and sets sfft_last_error) for a
+ * bad version, n not a power of two, or an (n,k) the algorithm cannot run;
+ * the reference aborts or runs out of bounds on the last two.
+ * fftw_optimization: FFTW_ESTIMATE (64) or FFTW_MEASURE (0); accepted for
+ * compatibility, no effect. */
+sfft_plan *sfft_make_plan(int n, int k, sfft_version version,
+                          int fftw_optimization);
+
+/* src/sfft.h:164 / src/sfft.cc:103-117.  Frees everything the plan owns. */
+void sfft_free_plan(sfft_plan *plan);
+
+/* src/sfft.h:165 / src/sfft.cc:119-137.  out (n values) is zero-filled and
+ * the located coefficients are written, as the reference does. */
+void sfft_exec(sfft_plan *plan, complex_t *in, complex_t *out);
+
+/* src/sfft.h:166 / src/sfft.cc:139-147.  All signals go to the GPU as one
+ * batch; RNG draws are consumed in signal order 0..num-1. */
+void sfft_exec_many(sfft_plan *plan, int num, complex_t **in,
+                    complex_t **out);
+
+/* ---- extensions --------------------------------------------------------- */
+
+/* Sparse result: the documented sfft_output (src/sfft.h:34-35,
+ * doc/usage.rst:64-90) as a C struct.  freq/val have room for capacity
+ * entries; count is set by the exec call.  Buffers may be host or device
+ * memory.  When count > capacity only capacity entries are written. */
+typedef struct sfft_output_c {
+  int64_t count;
+  int64_t capacity;
+  int32_t *freq;
+  complex_t *val;
+} sfft_output_c;
+
+typedef enum sfft_precision {
+  /* FP64 throughout; reproduces the reference's arithmetic order. */
+  SFFT_PRECISION_FP64 = 0,
+  /* FP32 bucketization and bucket FFTs (compensated FP32 sums), FP64 comb
+   * selection and FP64 estimation of the located values. */
+  SFFT_PRECISION_FP32_REFINE = 1
+} sfft_precision;
+
+/* Upper bound of sfft_output_c entries one exec of this plan can produce. */
+int64_t sfft_plan_output_capacity(const sfft_plan *plan);
+
+/* One signal, sparse result. Returns 0 on success, -1 on error. */
+int sfft_exec_sparse(sfft_plan *plan, const complex_t *in, sfft_output_c *out);
+
+/* num signals in one batch, sparse results. Returns 0 or -1. */
+int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
+                          sfft_output_c *out);
+
+/* Precision of later execs of this plan (default SFFT_PRECISION_FP64). */
+int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision);
+
+/* RNG streams.  By default a plan draws from the process-global glibc
+ * random() and drand48() streams exactly as the reference does (a caller's
+ * srand()/srand48() decide the draws).  Seeding gives the plan private
+ * streams equal to srandom(random_seed) / srand48(drand48_seed) but
+ * unaffected by other users of the global ones. */
+int sfft_plan_seed(sfft_plan *plan, unsigned int random_seed,
+                   long drand48_seed);
+
+/* HIP stream (hipStream_t) the plan's work is enqueued on; NULL = the
+ * library's own stream.  Exec calls return after their work completed. */
+int sfft_plan_set_stream(sfft_plan *plan, void *hip_stream);
+
+/* Values as the reference returns them (default, 1) or with the v1/v2
+ * conjugation of src/computefourier-1.0-2.0.cc:383-398 undone (0).  v3 is
+ * unaffected. */
+int sfft_plan_set_reference_conjugate(sfft_plan *plan, int enabled);
+
+/* Device bytes held by the plan and needed per signal of a batch. */
+int sfft_plan_memory(const sfft_plan *plan, int64_t *plan_bytes,
+                     int64_t *bytes_per_signal);
+
+/* Message of the last error on this thread ("" if none). */
+const char *sfft_last_error(void);
+
+/* Parameters the plan derived (for tests and tools).  Returns 0 or -1.
+ * p (10 ints): B_loc, B_est, B_thresh, W_comb, comb_loops, loc_loops,
+ * est_loops, thr_loops, w_loc, w_est.  For v3: W_Man, B_g1, B_g2, w_g1,
+ * w_g2, 0... */
+int sfft_plan_params(const sfft_plan *plan, int *p10);
+
+/* The same parameters for (n, k, version) without building a plan (host
+ * only, no GPU needed).  Returns 0 when sfft_make_plan would accept the
+ * triple, 1 when it would reject it (reason in sfft_last_error), -1 on a
+ * bad version. */
+int sfft_derive_params(int n, int k, sfft_version version, int *p10);
+
+/* Stage timing with HIP events recorded on the plan's stream around each
+ * kernel stage of every later exec (0 disables, any other value enables and
+ * resets the totals). */
+int sfft_plan_enable_timing(sfft_plan *plan, int enabled);
+
+/* Totals since timing was enabled: for stage s (0 comb, 1 fold+bucket FFT,
+ * 2 selection+votes, 3 estimation, 4 v3 loop) ms[s] = summed milliseconds
+ * and launches[s] = number of kernel launches timed. */
+int sfft_plan_stage_times(const sfft_plan *plan, double *ms5,
+                          int64_t *launches5);
+
+/* Diagnostics: the bucket values (after the bucket FFTs) of signal `signal`
+ * of the plan's last v1/v2 batch, all loops concatenated as the reference's
+ * x_samp (src/computefourier-1.0-2.0.cc:228-230): count entries into host
+ * memory.  Returns the number written or -1. */
+int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
+                           int64_t count);
+
+/* Host-side filter design of the library (src/filters.cc:70-160), for
+ * tests: w taps into time (capacity time_cap) and the n-point response into
+ * freq (may be NULL).  Returns w, or -1 when time_cap < w. */
+int sfft_design_filter(double lobefrac, double tolerance, int n, int b,
+                       complex_t *time, int time_cap, complex_t *freq);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -96,6 +96,7 @@ def lib():
         L.or_v12_plan_make.argtypes = [C.c_int, C.c_int, C.c_int]
         L.or_v12_plan_make.restype = C.POINTER(V12Plan)
         L.or_v12_plan_free.argtypes = [C.POINTER(V12Plan)]
+        L.or_v12_plan_set_filters.argtypes = [C.POINTER(V12Plan), P, P, P, P]
         L.or_v12_draw.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
         L.or_v12_exec.argtypes = [C.POINTER(V12Plan), P, P, P, P, P, C.POINTER(V12Trace)]
         L.or_v3_params_get.argtypes = [C.c_int, C.c_int, C.POINTER(V3Params)]
@@ -199,6 +200,13 @@ class V12:
                 "loc_freq": arr(h.loc_freq, self.n),
                 "est_time": arr(h.est_time, p.w_est),
                 "est_freq": arr(h.est_freq, self.n)}
+
+    def set_filters(self, loc_time, loc_freq, est_time, est_freq):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128)
+                for a in (loc_time, loc_freq, est_time, est_freq)]
+        assert len(arrs[0]) == self.p.w_loc and len(arrs[2]) == self.p.w_est
+        assert len(arrs[1]) == self.n and len(arrs[3]) == self.n
+        lib().or_v12_plan_set_filters(self._h, *[_p(a) for a in arrs])
 
     def draw(self):
         """RNG draws of one exec from the process-global glibc streams."""

@@ -13,6 +13,7 @@
 #include "sfft_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -139,9 +140,12 @@ static void fft_pow2_inplace(or_cplx *x, int n, int backward)
   double *wi = (double *)malloc(sizeof(double) * (size_t)half);
   double sgn = backward ? 1.0 : -1.0;
   for (int k = 0; k < half; k++) {
-    double ang = 2.0 * M_PI * (double)k / (double)n;
-    wr[k] = cos(ang);
-    wi[k] = sgn * sin(ang);
+    /* sincos() explicitly: compilers fuse sin+cos into it or not, and the
+     * two differ in the last bit for rare arguments */
+    double ang = 2.0 * M_PI * (double)k / (double)n, s, c;
+    sincos(ang, &s, &c);
+    wr[k] = c;
+    wi[k] = sgn * s;
   }
   for (int len = 2; len <= n; len <<= 1) {
     int h = len >> 1, stride = n / len;
@@ -444,6 +448,17 @@ or_v12_plan *or_v12_plan_make(int n, int k, int with_comb)
   return P;
 }
 
+void or_v12_plan_set_filters(or_v12_plan *P, const or_cplx *loc_time,
+                             const or_cplx *loc_freq, const or_cplx *est_time,
+                             const or_cplx *est_freq)
+{
+  size_t N = (size_t)P->p.n;
+  memcpy(P->loc_time, loc_time, sizeof(or_cplx) * (size_t)P->p.w_loc);
+  memcpy(P->loc_freq, loc_freq, sizeof(or_cplx) * N);
+  memcpy(P->est_time, est_time, sizeof(or_cplx) * (size_t)P->p.w_est);
+  memcpy(P->est_freq, est_freq, sizeof(or_cplx) * N);
+}
+
 void or_v12_plan_free(or_v12_plan *P)
 {
   if (!P) return;
@@ -717,12 +732,18 @@ int or_v12_exec_many(const or_v12_plan *P, int num, const or_cplx *const *xs,
                      const int *a, const int *ai, const int *comb_off,
                      or_cplx *const *outs, int threads)
 {
+  /* threads > 0: outs has num entries, one per signal.
+   * threads < 0: -threads workers and outs has one entry per worker (a
+   * timing mode that keeps the host footprint at one output per thread). */
   const int L = P->p.loc_loops + P->p.est_loops;
   const int cl = P->p.comb_loops > 0 ? P->p.comb_loops : 1;
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+  const int per_thread = threads < 0;
+  const int nt = per_thread ? -threads : threads;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
   for (int s = 0; s < num; s++)
     or_v12_exec(P, xs[s], a + (long)s * L, ai + (long)s * L,
-                comb_off + (long)s * cl, outs[s], NULL);
+                comb_off + (long)s * cl,
+                per_thread ? outs[omp_get_thread_num()] : outs[s], NULL);
   return 0;
 }
 

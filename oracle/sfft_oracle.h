@@ -72,6 +72,12 @@ typedef struct {
 
 or_v12_plan *or_v12_plan_make(int n, int k, int with_comb);
 void or_v12_plan_free(or_v12_plan *P);
+/* Replace the plan's filters by given ones (w_loc/w_est taps, n-point
+ * responses), so that a test can run the oracle with another design's
+ * filters and compare everything downstream bit for bit. */
+void or_v12_plan_set_filters(or_v12_plan *P, const or_cplx *loc_time,
+                             const or_cplx *loc_freq, const or_cplx *est_time,
+                             const or_cplx *est_freq);
 
 /* RNG draws of one v1/v2 exec, in reference order (cf12 465-474, 62):
  * loops x (a, ai) from random(), then comb_loops offsets from drand48(). */

@@ -1,0 +1,435 @@
+// extern "C" entry points: the reference API (src/sfft.h:158-167) plus the
+// sparse / precision / RNG / stream extensions declared in include/sfft.h.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace sfa {
+const char *last_error();
+}
+
+using namespace sfa;
+
+namespace {
+
+Plan *get(const sfft_plan *plan)
+{
+  if (!plan || !plan->data) {
+    set_error("null plan");
+    return nullptr;
+  }
+  return reinterpret_cast<Plan *>(plan->data);
+}
+
+bool is_device(const void *p)
+{
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// Signals per GPU batch: bounded by workspace and, for host inputs, by the
+// staging buffer (whole signals are copied to the device).
+int chunk_signals(const Plan *P, bool host_inputs)
+{
+  const int64_t budget = 4LL << 30;
+  int64_t per = std::max<int64_t>(1, P->bytes_per_signal());
+  if (host_inputs) per += (int64_t)P->n * 16;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(4096, budget / per));
+}
+
+// Runs num signals.  Exactly one of (dense, sparse) is non-null.
+int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
+        sfft_output_c *sparse)
+{
+  if (num <= 0) return 0;
+  if (P->version == SFFT_VERSION_3) {
+    set_error("sFFT v3 is not available in this build yet");
+    return -1;
+  }
+  const int64_t n = P->n;
+  const int64_t cap = P->output_capacity();
+  bool host_in = false;
+  std::vector<char> in_dev(num);
+  for (int s = 0; s < num; s++) {
+    if (!in[s]) {
+      set_error("null input signal");
+      return -1;
+    }
+    in_dev[s] = is_device(in[s]);
+    host_in |= !in_dev[s];
+  }
+  const int chunk = chunk_signals(P, host_in);
+  hipStream_t st = P->stream;
+  for (int c0 = 0; c0 < num; c0 += chunk) {
+    const int S = std::min(chunk, num - c0);
+    std::vector<const double2 *> ins(S);
+    std::vector<int32_t *> fr(S, nullptr);
+    std::vector<double2 *> va(S, nullptr), de(S, nullptr);
+    std::vector<int64_t> counts(S, 0);
+    std::vector<char> via_ws(S, 0);  // results staged in the plan's buffers
+    // inputs
+    int staged = 0;
+    for (int s = 0; s < S; s++) staged += !in_dev[c0 + s];
+    if (staged && !P->ws_stage.ensure((size_t)staged * n * 16)) return -1;
+    int slot = 0;
+    for (int s = 0; s < S; s++) {
+      const complex_t *src = in[c0 + s];
+      if (in_dev[c0 + s]) {
+        ins[s] = reinterpret_cast<const double2 *>(src);
+      } else {
+        double2 *dst = P->ws_stage.as<double2>() + (size_t)slot++ * n;
+        if (hipMemcpyAsync(dst, src, n * 16, hipMemcpyHostToDevice, st) != hipSuccess) {
+          set_error("input copy failed");
+          return -1;
+        }
+        ins[s] = dst;
+      }
+    }
+    // outputs
+    int nws = 0;
+    for (int s = 0; s < S; s++) {
+      if (dense) {
+        complex_t *o = dense[c0 + s];
+        if (!o) {
+          set_error("null output signal");
+          return -1;
+        }
+        if (is_device(o)) {
+          de[s] = reinterpret_cast<double2 *>(o);
+          if (k::zero_dense(de[s], n, st) != 0) return -1;
+        } else {
+          via_ws[s] = 1;
+          nws++;
+        }
+      } else {
+        sfft_output_c &o = sparse[c0 + s];
+        if (o.freq && o.val && o.capacity >= cap && is_device(o.freq) &&
+            is_device(o.val)) {
+          fr[s] = o.freq;
+          va[s] = reinterpret_cast<double2 *>(o.val);
+        } else {
+          via_ws[s] = 1;
+          nws++;
+        }
+      }
+    }
+    if (nws) {
+      if (!P->ws_out_freq.ensure((size_t)nws * cap * 4) ||
+          !P->ws_out_val.ensure((size_t)nws * cap * 16))
+        return -1;
+      int q = 0;
+      for (int s = 0; s < S; s++) {
+        if (!via_ws[s]) continue;
+        fr[s] = P->ws_out_freq.as<int32_t>() + (size_t)q * cap;
+        va[s] = P->ws_out_val.as<double2>() + (size_t)q * cap;
+        q++;
+      }
+    }
+    BatchIO io;
+    io.S = S;
+    io.in = ins.data();
+    const bool any_sparse = nws > 0 || !dense;
+    io.freq = any_sparse ? fr.data() : nullptr;
+    io.val = any_sparse ? va.data() : nullptr;
+    io.dense = dense ? de.data() : nullptr;
+    io.cap = cap;
+    io.counts = counts.data();
+    if (exec_batch(P, io) != 0) return -1;
+    // results that went through the plan's buffers
+    for (int s = 0; s < S; s++) {
+      const int64_t cnt = std::min(counts[s], cap);
+      if (dense) {
+        if (!via_ws[s]) continue;
+        std::vector<int32_t> hf(cnt);
+        std::vector<double2> hv(cnt);
+        if (cnt && (hipMemcpy(hf.data(), fr[s], cnt * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(hv.data(), va[s], cnt * 16, hipMemcpyDeviceToHost) != hipSuccess)) {
+          set_error("result copy failed");
+          return -1;
+        }
+        double2 *o = reinterpret_cast<double2 *>(dense[c0 + s]);
+        std::memset(o, 0, (size_t)n * 16);  // src/sfft.cc:121-123
+        for (int64_t j = 0; j < cnt; j++) o[hf[j]] = hv[j];
+      } else {
+        sfft_output_c &o = sparse[c0 + s];
+        o.count = counts[s];
+        if (!via_ws[s]) continue;
+        const int64_t m = std::min<int64_t>(cnt, std::max<int64_t>(0, o.capacity));
+        if (m && o.freq &&
+            hipMemcpy(o.freq, fr[s], m * 4, hipMemcpyDefault) != hipSuccess) {
+          set_error("result copy failed");
+          return -1;
+        }
+        if (m && o.val &&
+            hipMemcpy(o.val, va[s], m * 16, hipMemcpyDefault) != hipSuccess) {
+          set_error("result copy failed");
+          return -1;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *sfft_malloc(size_t s)
+{
+  void *p = nullptr;
+  if (posix_memalign(&p, 16, s ? s : 16) != 0) return nullptr;
+  return p;
+}
+
+void sfft_free(void *p) { free(p); }
+
+sfft_plan *sfft_make_plan(int n, int k, sfft_version version,
+                          int fftw_optimization)
+{
+  set_error("");
+  (void)fftw_optimization;
+  Plan *P = make_plan(n, k, version);
+  if (!P) return nullptr;
+  sfft_plan *plan = (sfft_plan *)std::malloc(sizeof(sfft_plan));
+  if (!plan) {
+    free_plan(P);
+    set_error("out of host memory");
+    return nullptr;
+  }
+  plan->version = version;
+  plan->n = (unsigned)n;
+  plan->k = (unsigned)k;
+  plan->data = P;
+  return plan;
+}
+
+void sfft_free_plan(sfft_plan *plan)
+{
+  if (!plan) return;
+  free_plan(reinterpret_cast<Plan *>(plan->data));
+  std::free(plan);
+}
+
+void sfft_exec(sfft_plan *plan, complex_t *in, complex_t *out)
+{
+  set_error("");
+  Plan *P = get(plan);
+  if (!P) return;
+  const complex_t *ins[1] = {in};
+  complex_t *outs[1] = {out};
+  (void)run(P, 1, ins, outs, nullptr);
+}
+
+void sfft_exec_many(sfft_plan *plan, int num, complex_t **in, complex_t **out)
+{
+  set_error("");
+  Plan *P = get(plan);
+  if (!P) return;
+  (void)run(P, num, in, out, nullptr);
+}
+
+int64_t sfft_plan_output_capacity(const sfft_plan *plan)
+{
+  Plan *P = get(plan);
+  return P ? P->output_capacity() : -1;
+}
+
+int sfft_exec_sparse(sfft_plan *plan, const complex_t *in, sfft_output_c *out)
+{
+  set_error("");
+  Plan *P = get(plan);
+  if (!P || !out) return -1;
+  const complex_t *ins[1] = {in};
+  return run(P, 1, ins, nullptr, out);
+}
+
+int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
+                          sfft_output_c *out)
+{
+  set_error("");
+  Plan *P = get(plan);
+  if (!P || !out) return -1;
+  return run(P, num, in, nullptr, out);
+}
+
+int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if (precision != SFFT_PRECISION_FP64 && precision != SFFT_PRECISION_FP32_REFINE) {
+    set_error("unknown precision");
+    return -1;
+  }
+  P->precision = precision;
+  return 0;
+}
+
+int sfft_plan_seed(sfft_plan *plan, unsigned int random_seed, long drand48_seed)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  P->rng.seed(random_seed, drand48_seed);
+  return 0;
+}
+
+int sfft_plan_set_stream(sfft_plan *plan, void *hip_stream)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  (void)hipStreamSynchronize(P->stream);
+  if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
+  P->own_stream = false;
+  P->stream = reinterpret_cast<hipStream_t>(hip_stream);
+  if (!P->stream) {
+    if (hipStreamCreate(&P->stream) != hipSuccess) {
+      set_error("hipStreamCreate failed");
+      return -1;
+    }
+    P->own_stream = true;
+  }
+  return 0;
+}
+
+int sfft_plan_set_reference_conjugate(sfft_plan *plan, int enabled)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  P->ref_conj = enabled != 0;
+  return 0;
+}
+
+int sfft_plan_memory(const sfft_plan *plan, int64_t *plan_bytes,
+                     int64_t *bytes_per_signal)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if (plan_bytes) *plan_bytes = P->plan_bytes();
+  if (bytes_per_signal) *bytes_per_signal = P->bytes_per_signal();
+  return 0;
+}
+
+const char *sfft_last_error(void) { return sfa::last_error(); }
+
+int sfft_plan_params(const sfft_plan *plan, int *p10)
+{
+  Plan *P = get(plan);
+  if (!P || !p10) return -1;
+  std::memset(p10, 0, 10 * sizeof(int));
+  if (P->version == SFFT_VERSION_3) {
+    const V3Params &q = P->v3;
+    int v[10] = {q.W_Man, q.B_g1, q.B_g2, q.w_g1, q.w_g2, 0, 0, 0, 0, 0};
+    std::memcpy(p10, v, sizeof(v));
+  } else {
+    const V12Params &q = P->v12;
+    int v[10] = {q.B_loc, q.B_est, q.B_thresh, q.W_comb, q.comb_loops,
+                 q.loc_loops, q.est_loops, q.thr_loops, q.w_loc, q.w_est};
+    std::memcpy(p10, v, sizeof(v));
+  }
+  return 0;
+}
+
+int sfft_derive_params(int n, int k, sfft_version version, int *p10)
+{
+  if (!p10) return -1;
+  std::memset(p10, 0, 10 * sizeof(int));
+  std::string why;
+  if (version == SFFT_VERSION_3) {
+    const V3Params q = derive_v3(n, k);
+    int v[10] = {q.W_Man, q.B_g1, q.B_g2, q.w_g1, q.w_g2, 0, 0, 0, 0, 0};
+    std::memcpy(p10, v, sizeof(v));
+    why = (n > 0 && (n & (n - 1)) == 0) ? check_v3(q) : "n must be a power of two";
+  } else if (version == SFFT_VERSION_1 || version == SFFT_VERSION_2) {
+    const V12Params q = derive_v12(n, k, version == SFFT_VERSION_2);
+    int v[10] = {q.B_loc, q.B_est, q.B_thresh, q.W_comb, q.comb_loops,
+                 q.loc_loops, q.est_loops, q.thr_loops, q.w_loc, q.w_est};
+    std::memcpy(p10, v, sizeof(v));
+    why = (n > 0 && (n & (n - 1)) == 0) ? check_v12(q) : "n must be a power of two";
+  } else {
+    set_error("unknown sfft_version");
+    return -1;
+  }
+  if (!why.empty()) {
+    set_error(why);
+    return 1;
+  }
+  return 0;
+}
+
+int sfft_plan_enable_timing(sfft_plan *plan, int enabled)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  P->timing = enabled != 0;
+  for (int i = 0; i < Plan::kStages; i++) {
+    P->stage_ms[i] = 0;
+    P->stage_launches[i] = 0;
+  }
+  return 0;
+}
+
+int sfft_plan_stage_times(const sfft_plan *plan, double *ms5, int64_t *launches5)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  for (int i = 0; i < Plan::kStages; i++) {
+    if (ms5) ms5[i] = P->stage_ms[i];
+    if (launches5) launches5[i] = P->stage_launches[i];
+  }
+  return 0;
+}
+
+int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
+                           int64_t count)
+{
+  Plan *P = get(plan);
+  if (!P || !dst || P->version == SFFT_VERSION_3 || signal < 0) return -1;
+  const int64_t xs = P->v12.x_samp_size();
+  if (signal >= P->last_S) {
+    set_error("no such signal in the last batch");
+    return -1;
+  }
+  const bool fp32 = P->last_fp32;
+  // loops [0, n64) are FP64 buffers, the rest FP32 (see exec_batch)
+  const int64_t n64 = !fp32 ? xs : (!P->v12.comb ? (int64_t)P->v12.loc_loops * P->v12.B_loc : 0);
+  const int64_t m = std::min(count, xs);
+  std::vector<double2> t64(std::min(m, n64));
+  std::vector<float2> t32(std::max<int64_t>(0, m - n64));
+  if ((!t64.empty() &&
+       hipMemcpy(t64.data(), P->ws_xsamp.as<double2>() + (size_t)signal * xs,
+                 t64.size() * 16, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (!t32.empty() &&
+       hipMemcpy(t32.data(), P->ws_xsamp32.as<float2>() + (size_t)signal * xs + n64,
+                 t32.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)) {
+    set_error("bucket copy failed");
+    return -1;
+  }
+  for (size_t i = 0; i < t64.size(); i++) dst[i] = complex_t(t64[i].x, t64[i].y);
+  for (size_t i = 0; i < t32.size(); i++) dst[n64 + i] = complex_t(t32[i].x, t32[i].y);
+  return m;
+}
+
+int sfft_design_filter(double lobefrac, double tolerance, int n, int b,
+                       complex_t *time, int time_cap, complex_t *freq)
+{
+  const int w = dolph_chebyshev_width(lobefrac, tolerance);
+  if (!time || time_cap < w || n <= 0) {
+    set_error("time buffer too small");
+    return -1;
+  }
+  HostFilter f = design_filter(lobefrac, tolerance, n, b);
+  for (int i = 0; i < w; i++) time[i] = f.time[i];
+  if (freq)
+    for (int i = 0; i < n; i++) freq[i] = f.freq[i];
+  return w;
+}
+
+}  // extern "C"

@@ -1,0 +1,410 @@
+// Plan construction (device-resident tables), RNG draws and batch execution
+// of sFFT v1/v2 on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace sfa {
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+#define HIP_OK(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return -1;                                                            \
+    }                                                                       \
+  } while (0)
+
+// ---------------------------------------------------------------- RNG
+int gcd_int(int a, int b)
+{
+  // gcd(0, n) = n, as src/utils.cc:41-46 (which forces the first draw).
+  while (a % b != 0) {
+    int t = a % b;
+    a = b;
+    b = t;
+  }
+  return b;
+}
+
+int mod_inverse_int(int a, int n)
+{
+  // Extended Euclid on (n, a); result in [0, n).
+  int64_t r0 = n, r1 = a, s0 = 0, s1 = 1;
+  while (r1 != 0) {
+    int64_t q = r0 / r1, t = r0 - q * r1;
+    r0 = r1;
+    r1 = t;
+    t = s0 - q * s1;
+    s0 = s1;
+    s1 = t;
+  }
+  int64_t v = s0 % n;
+  if (v < 0) v += n;
+  return (int)v;
+}
+
+void Rng::seed(unsigned rs, long ds)
+{
+  std::memset(&rd, 0, sizeof(rd));
+  std::memset(state, 0, sizeof(state));
+  initstate_r(rs, state, sizeof(state), &rd);  // TYPE_3, as srandom()
+  srand48_r(ds, &dd);
+  seeded = true;
+}
+
+long Rng::next_random()
+{
+  if (!seeded) return ::random();
+  int32_t r = 0;
+  random_r(&rd, &r);
+  return r;
+}
+
+double Rng::next_drand48()
+{
+  if (!seeded) return ::drand48();
+  double r = 0;
+  drand48_r(&dd, &r);
+  return r;
+}
+
+// ---------------------------------------------------------------- buffers
+bool DevBuf::ensure(size_t nbytes)
+{
+  if (nbytes <= bytes && p) return true;
+  release();
+  if (nbytes == 0) nbytes = 256;
+  if (hipMalloc(&p, nbytes) != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    set_error("hipMalloc of " + std::to_string(nbytes) + " bytes failed");
+    return false;
+  }
+  bytes = nbytes;
+  return true;
+}
+
+void DevBuf::release()
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+template <typename T>
+static bool upload(DevBuf &b, const std::vector<T> &v)
+{
+  if (!b.ensure(v.size() * sizeof(T))) return false;
+  if (hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    set_error("table upload failed");
+    return false;
+  }
+  return true;
+}
+
+static std::vector<double2> to_d2(const std::vector<cd> &v, size_t cnt)
+{
+  std::vector<double2> o(cnt);
+  for (size_t i = 0; i < cnt; i++) o[i] = make_double2(v[i].real(), v[i].imag());
+  return o;
+}
+
+static std::vector<float2> to_f2(const std::vector<cd> &v, size_t cnt)
+{
+  std::vector<float2> o(cnt);
+  for (size_t i = 0; i < cnt; i++)
+    o[i] = make_float2((float)v[i].real(), (float)v[i].imag());
+  return o;
+}
+
+// Estimation window of one filter: win[d + m/2] = freq[(n - d) mod n] for
+// d = -m/2..m/2 (m = n/B); estimate_values reads only these
+// (src/computefourier-1.0-2.0.cc:371-385).
+static std::vector<double2> est_window(const std::vector<cd> &freq, int n, int B)
+{
+  const int m = n / B;
+  std::vector<double2> w(m + 1);
+  for (int d = -(m / 2); d <= m / 2; d++) {
+    const cd v = freq[(size_t)(((int64_t)n - d) % n)];
+    w[d + m / 2] = make_double2(v.real(), v.imag());
+  }
+  return w;
+}
+
+// ---------------------------------------------------------------- plan
+int64_t Plan::output_capacity() const
+{
+  if (version == SFFT_VERSION_3) return 8LL * v3.k + 4LL * v3.W_Man + 64;
+  const V12Params &p = v12;
+  if (p.comb) return (int64_t)p.comb_loops * p.B_thresh * (p.n / p.W_comb);
+  // v1: every hit is selected in >= thr of the location loops.
+  const int64_t sel = (int64_t)p.loc_loops * p.B_thresh * (p.n / p.B_loc);
+  return std::min<int64_t>(p.n, sel / std::max(1, p.thr_loops) + 1);
+}
+
+int64_t Plan::plan_bytes() const
+{
+  return (int64_t)(taps_loc.bytes + taps_est.bytes + taps_loc32.bytes +
+                   taps_est32.bytes + win_loc.bytes + win_est.bytes +
+                   tw64.bytes + tw32.bytes + v3_time1.bytes + v3_time2.bytes +
+                   v3_freq1.bytes + v3_freq2.bytes);
+}
+
+int64_t Plan::bytes_per_signal() const
+{
+  if (version == SFFT_VERSION_3) return 64LL * 1024 + output_capacity() * 24;
+  const V12Params &p = v12;
+  return p.x_samp_size() * 16 + (int64_t)p.comb_loops * p.B_thresh * 4 +
+         (int64_t)p.loc_loops * p.B_thresh * 4 + output_capacity() * (4 + 16 + 4) +
+         (2 * p.loops() + p.comb_loops + 8) * 4;
+}
+
+Plan *make_plan(int n, int k, sfft_version v)
+{
+  if (v != SFFT_VERSION_1 && v != SFFT_VERSION_2 && v != SFFT_VERSION_3) {
+    set_error("unknown sfft_version");
+    return nullptr;
+  }
+  if (n <= 0 || (n & (n - 1)) != 0) {
+    set_error("n must be a power of two");
+    return nullptr;
+  }
+  Plan *P = new Plan();
+  P->version = v;
+  P->n = n;
+  P->k = k;
+  if (v == SFFT_VERSION_3) {
+    P->v3 = derive_v3(n, k);
+    std::string why = check_v3(P->v3);
+    if (!why.empty()) {
+      set_error(why);
+      delete P;
+      return nullptr;
+    }
+    set_error("sFFT v3 is not available in this build yet");
+    delete P;
+    return nullptr;
+  }
+  P->v12 = derive_v12(n, k, v == SFFT_VERSION_2);
+  std::string why = check_v12(P->v12);
+  if (!why.empty()) {
+    set_error(why);
+    delete P;
+    return nullptr;
+  }
+  const V12Params &p = P->v12;
+  HostFilter fl = design_filter(0.5 / p.BB_loc, p.tol_loc, p.n, p.b_loc);
+  HostFilter fe = design_filter(0.5 / p.BB_est, p.tol_est, p.n, p.b_est);
+  int nmax = std::max(p.B_loc, p.B_est);
+  if (p.comb) nmax = std::max(nmax, p.W_comb);
+  std::vector<cd> tw(nmax / 2);
+  for (int j = 0; j < nmax / 2; j++) {
+    // sincos() explicitly: whether a compiler fuses sin+cos into it varies,
+    // and the two differ in the last bit for rare arguments.
+    const double ang = 2.0 * M_PI * (double)j / (double)nmax;
+    double s, c;
+    ::sincos(ang, &s, &c);
+    tw[j] = cd(c, -s);
+  }
+  P->tw_nmax = nmax;
+  bool ok = upload(P->taps_loc, to_d2(fl.time, fl.time.size())) &&
+            upload(P->taps_est, to_d2(fe.time, fe.time.size())) &&
+            upload(P->taps_loc32, to_f2(fl.time, fl.time.size())) &&
+            upload(P->taps_est32, to_f2(fe.time, fe.time.size())) &&
+            upload(P->win_loc, est_window(fl.freq, p.n, p.B_loc)) &&
+            upload(P->win_est, est_window(fe.freq, p.n, p.B_est)) &&
+            upload(P->tw64, to_d2(tw, tw.size())) &&
+            upload(P->tw32, to_f2(tw, tw.size()));
+  if (!ok || hipStreamCreate(&P->stream) != hipSuccess) {
+    if (ok) set_error("hipStreamCreate failed");
+    free_plan(P);
+    return nullptr;
+  }
+  P->own_stream = true;
+  return P;
+}
+
+void free_plan(Plan *P)
+{
+  if (!P) return;
+  if (P->stream) (void)hipStreamSynchronize(P->stream);
+  DevBuf *bufs[] = {&P->taps_loc, &P->taps_est, &P->taps_loc32, &P->taps_est32,
+                    &P->win_loc, &P->win_est, &P->tw64, &P->tw32,
+                    &P->v3_time1, &P->v3_time2, &P->v3_freq1, &P->v3_freq2,
+                    &P->ws_params, &P->ws_xsamp, &P->ws_xsamp32, &P->ws_approved,
+                    &P->ws_numcomb, &P->ws_J, &P->ws_hits,
+                    &P->ws_ptrs, &P->ws_out_freq, &P->ws_out_val,
+                    &P->ws_count, &P->ws_stage, &P->ws_v3};
+  for (DevBuf *b : bufs) b->release();
+  if (P->ev_made)
+    for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
+  if (P->pinned) (void)hipHostFree(P->pinned);
+  if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+}
+
+// ---------------------------------------------------------------- exec
+static int ensure_pinned(Plan *P, size_t bytes)
+{
+  if (bytes <= P->pinned_bytes) return 0;
+  if (P->pinned) (void)hipHostFree(P->pinned);
+  P->pinned = nullptr;
+  P->pinned_bytes = 0;
+  HIP_OK(hipHostMalloc(&P->pinned, bytes, hipHostMallocDefault));
+  P->pinned_bytes = bytes;
+  return 0;
+}
+
+int exec_batch(Plan *P, BatchIO &io)
+{
+  const int S = io.S;
+  const V12Params &p = P->v12;
+  const int L = p.loops(), cl = p.comb ? p.comb_loops : 0;
+  const int n = p.n, num = p.B_thresh;
+  const int stride = 2 * L + std::max(cl, 1);
+  const int64_t cap = P->output_capacity();
+  const int64_t xs = p.x_samp_size();
+  const bool fp32 = P->precision == SFFT_PRECISION_FP32_REFINE;
+
+  // Pinned staging: params | 4 pointer tables | counts.
+  const size_t param_bytes = ((size_t)S * stride * sizeof(int) + 255) & ~size_t(255);
+  const size_t tab_bytes = ((size_t)S * sizeof(void *) + 255) & ~size_t(255);
+  const size_t cnt_bytes = (size_t)S * sizeof(int64_t);
+  if (ensure_pinned(P, param_bytes + 4 * tab_bytes + cnt_bytes) != 0) return -1;
+  char *pin = reinterpret_cast<char *>(P->pinned);
+  int *hp = reinterpret_cast<int *>(pin);
+  const void **t_in = reinterpret_cast<const void **>(pin + param_bytes);
+  const void **t_freq = reinterpret_cast<const void **>(pin + param_bytes + tab_bytes);
+  const void **t_val = reinterpret_cast<const void **>(pin + param_bytes + 2 * tab_bytes);
+  const void **t_dense = reinterpret_cast<const void **>(pin + param_bytes + 3 * tab_bytes);
+  int64_t *h_counts = reinterpret_cast<int64_t *>(pin + param_bytes + 4 * tab_bytes);
+
+  // RNG draws in signal order (src/computefourier-1.0-2.0.cc:465-474, 62).
+  const int sigma = p.comb ? n / p.W_comb : 1;
+  for (int s = 0; s < S; s++) {
+    int *row = hp + (size_t)s * stride;
+    for (int l = 0; l < L; l++) {
+      int a = 0;
+      while (gcd_int(a, n) != 1) a = (int)(P->rng.next_random() % n);
+      row[l] = a;
+      row[L + l] = mod_inverse_int(a, n);
+    }
+    for (int c = 0; c < cl; c++)
+      row[2 * L + c] = (int)(unsigned)std::floor(P->rng.next_drand48() * sigma);
+  }
+  // v1 keeps its location loops in FP64 under fp32 (selection-critical)
+  const bool loc64 = fp32 && !p.comb;
+  const bool need64 = !fp32 || loc64;
+  const bool want_sparse = io.freq != nullptr || io.val != nullptr;
+  const bool want_dense = io.dense != nullptr;
+  for (int s = 0; s < S; s++) {
+    t_in[s] = io.in[s];
+    t_freq[s] = io.freq ? io.freq[s] : nullptr;
+    t_val[s] = io.val ? io.val[s] : nullptr;
+    t_dense[s] = io.dense ? io.dense[s] : nullptr;
+  }
+
+  if (!P->ws_params.ensure(param_bytes) || !P->ws_ptrs.ensure(4 * tab_bytes) ||
+      (need64 && !P->ws_xsamp.ensure((size_t)S * xs * 16)) ||
+      (fp32 && !P->ws_xsamp32.ensure((size_t)S * xs * 8)) ||
+      !P->ws_approved.ensure((size_t)S * std::max(cl, 1) * num * 4) ||
+      !P->ws_numcomb.ensure((size_t)S * 4) ||
+      !P->ws_J.ensure((size_t)S * p.loc_loops * num * 4) ||
+      !P->ws_hits.ensure((size_t)S * (p.comb ? 1 : cap) * 4) ||
+      !P->ws_count.ensure((size_t)S * 8))
+    return -1;
+  hipStream_t st = P->stream;
+  HIP_OK(hipMemcpyAsync(P->ws_params.p, hp, (size_t)S * stride * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(P->ws_ptrs.p, pin + param_bytes, 4 * tab_bytes,
+                        hipMemcpyHostToDevice, st));
+  char *dtab = reinterpret_cast<char *>(P->ws_ptrs.p);
+
+  k::V12Launch a{};
+  a.n = n;
+  a.S = S;
+  a.L = L;
+  a.Lloc = p.loc_loops;
+  a.B_loc = p.B_loc;
+  a.B_est = p.B_est;
+  a.w_loc = p.w_loc;
+  a.w_est = p.w_est;
+  a.num = num;
+  a.W = p.W_comb;
+  a.comb_loops = cl;
+  a.thr = p.thr_loops;
+  a.x = reinterpret_cast<const double2 *const *>(dtab);
+  a.params = P->ws_params.as<const int>();
+  a.taps_loc = P->taps_loc.as<const double2>();
+  a.taps_est = P->taps_est.as<const double2>();
+  a.taps_loc32 = P->taps_loc32.as<const float2>();
+  a.taps_est32 = P->taps_est32.as<const float2>();
+  a.win_loc = P->win_loc.as<const double2>();
+  a.win_est = P->win_est.as<const double2>();
+  a.tw64 = P->tw64.as<const double2>();
+  a.tw32 = P->tw32.as<const float2>();
+  a.tw_nmax = P->tw_nmax;
+  a.fp32 = fp32;
+  a.loc64 = loc64;
+  a.ref_conj = P->ref_conj;
+  a.xs64 = need64 ? P->ws_xsamp.as<double2>() : nullptr;
+  a.xs32 = fp32 ? P->ws_xsamp32.as<float2>() : nullptr;
+  a.xs = xs;
+  a.approved = P->ws_approved.as<int>();
+  a.num_comb = P->ws_numcomb.as<int>();
+  a.J = P->ws_J.as<int>();
+  a.hits = P->ws_hits.as<int>();
+  a.hits_cap = cap;
+  a.count = P->ws_count.as<int64_t>();
+  a.o_freq = want_sparse ? reinterpret_cast<int32_t *const *>(dtab + tab_bytes) : nullptr;
+  a.o_val = want_sparse ? reinterpret_cast<double2 *const *>(dtab + 2 * tab_bytes) : nullptr;
+  a.o_dense = want_dense ? reinterpret_cast<double2 *const *>(dtab + 3 * tab_bytes) : nullptr;
+  a.o_cap = io.cap;
+  a.est_cap = cap;
+
+  if (P->timing && !P->ev_made) {
+    for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
+    P->ev_made = true;
+  }
+  auto mark = [&](int i) {
+    if (P->timing) (void)hipEventRecord(P->ev[i], st);
+  };
+  mark(0);
+  if (p.comb && k::comb(a, st) != 0) return -1;
+  mark(1);
+  P->last_fp32 = fp32;
+  P->last_S = S;
+  if (k::fold_fft(a, st) != 0) return -1;
+  mark(2);
+  if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
+  mark(3);
+  if (k::estimate(a, st) != 0) return -1;
+  mark(4);
+  HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (P->timing) {
+    const int launches[4] = {p.comb ? (cl > 1 ? 2 : 1) : 0, loc64 ? 2 : 1,
+                             p.comb ? 0 : 2, 1};
+    for (int i = 0; i < 4; i++) {
+      float ms = 0;
+      if (launches[i] && hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]) == hipSuccess) {
+        P->stage_ms[i] += ms;
+        P->stage_launches[i] += launches[i];
+      }
+    }
+  }
+  if (io.counts)
+    for (int s = 0; s < S; s++) io.counts[s] = h_counts[s];
+  return 0;
+}
+
+}  // namespace sfa

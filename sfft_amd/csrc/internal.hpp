@@ -1,0 +1,194 @@
+// Internal types of libsfft_amd: plan parameters, host filter design, the
+// plan object with its device-resident tables, and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/sfft.h"
+
+namespace sfa {
+
+using cd = std::complex<double>;
+
+// Largest bucket / comb / Mansour transform a single workgroup keeps in LDS.
+constexpr int kMaxLdsFft64 = 8192;
+// Largest number of loops (location + estimation) the median network takes.
+constexpr int kMaxLoops = 32;
+
+void set_error(const std::string &msg);
+
+// ---------------------------------------------------------------- params
+// sFFT v1/v2 sizes (src/sfft.cc:298-372 with src/parameters.cc).
+struct V12Params {
+  int n = 0, k = 0;
+  bool comb = false;
+  double Bcst_loc = 1, Bcst_est = 1, Comb_cst = 2, tol_loc = 1e-8, tol_est = 1e-8;
+  int loc_loops = 4, est_loops = 16, thr_loops = 3, comb_loops = 1;
+  double BB_loc = 0, BB_est = 0;
+  int b_loc = 0, b_est = 0, B_loc = 0, B_est = 0, B_thresh = 0, W_comb = 0;
+  int w_loc = 0, w_est = 0;
+  int loops() const { return loc_loops + est_loops; }
+  int64_t x_samp_size() const {
+    return (int64_t)loc_loops * B_loc + (int64_t)est_loops * B_est;
+  }
+};
+
+// sFFT v3 sizes (src/sfft.cc:506-560).
+struct V3Params {
+  int n = 0, k = 0;
+  int W_Man = 0, B_g1 = 0, B_g2 = 0, w_g1 = 0, w_g2 = 0, b_g1 = 0, b_g2 = 0;
+  double BB1 = 0, BB2 = 0;
+};
+
+int floor_to_pow2(double x);
+int dolph_chebyshev_width(double lobefrac, double tolerance);
+V12Params derive_v12(int n, int k, bool comb);
+V3Params derive_v3(int n, int k);
+// Empty string when the plan can run, else the reason.
+std::string check_v12(const V12Params &p);
+std::string check_v3(const V3Params &p);
+
+// ---------------------------------------------------------------- host FFT
+// In-place unnormalised DFT, e^{-2 pi i jk/n} (forward) or e^{+..}.
+void host_fft(cd *x, int n, bool backward);
+
+// ---------------------------------------------------------------- filters
+struct HostFilter {
+  std::vector<cd> time;  // w taps
+  std::vector<cd> freq;  // n-point response
+};
+// Dolph-Chebyshev window flattened over b bins (src/filters.cc:70-160).
+HostFilter design_filter(double lobefrac, double tolerance, int n, int b);
+
+// ---------------------------------------------------------------- RNG
+// The reference's random() / drand48() draws, from the process-global glibc
+// streams or from private streams seeded like srandom()/srand48().
+struct Rng {
+  bool seeded = false;
+  struct random_data rd;
+  char state[128];
+  struct drand48_data dd;
+  long next_random();
+  double next_drand48();
+  void seed(unsigned rs, long ds);
+};
+int gcd_int(int a, int b);
+int mod_inverse_int(int a, int n);
+
+// ---------------------------------------------------------------- device
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t nbytes);  // grow-only
+  void release();
+  template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Plan {
+  sfft_version version = SFFT_VERSION_1;
+  int n = 0, k = 0;
+  V12Params v12;
+  V3Params v3;
+  sfft_precision precision = SFFT_PRECISION_FP64;
+  bool last_fp32 = false;  // precision of the buckets in ws_xsamp
+  bool ref_conj = true;
+  Rng rng;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+
+  // v1/v2 device tables
+  DevBuf taps_loc, taps_est;      // double2[w]
+  DevBuf taps_loc32, taps_est32;  // float2[w]
+  DevBuf win_loc, win_est;        // double2[m+1]: freq[(n-d) mod n], d = -m/2..m/2
+  DevBuf tw64, tw32;              // e^{-2 pi i j / Nmax}, j < Nmax/2
+  int tw_nmax = 0;
+
+  // v3 device tables
+  DevBuf v3_time1, v3_time2;  // double2[w_g*]
+  DevBuf v3_freq1, v3_freq2;  // double2[n] (full responses, read sparsely)
+
+  // per-exec workspace (grow-only)
+  DevBuf ws_params, ws_xsamp, ws_xsamp32, ws_approved, ws_numcomb, ws_J,
+      ws_hits, ws_ptrs, ws_out_freq, ws_out_val, ws_count, ws_stage, ws_v3;
+  int last_S = 0;  // signals in the last batch (for diagnostics)
+  std::vector<char> host_stage;
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+
+  // stage timing (hipEvents on the plan's stream)
+  static constexpr int kStages = 5;
+  bool timing = false;
+  double stage_ms[kStages] = {0, 0, 0, 0, 0};
+  int64_t stage_launches[kStages] = {0, 0, 0, 0, 0};
+  hipEvent_t ev[kStages + 1] = {};
+  bool ev_made = false;
+
+  int64_t output_capacity() const;
+  int64_t plan_bytes() const;
+  int64_t bytes_per_signal() const;
+};
+
+Plan *make_plan(int n, int k, sfft_version v);
+void free_plan(Plan *P);
+
+// One batch of signals.  Every array here is a host array of S device
+// pointers; a null freq/val/dense array (or entry) means "not wanted".
+struct BatchIO {
+  int S = 0;
+  const double2 *const *in = nullptr;
+  int32_t *const *freq = nullptr;   // sparse frequencies, cap entries each
+  double2 *const *val = nullptr;    // sparse values, cap entries each
+  double2 *const *dense = nullptr;  // n values each, already zero-filled
+  int64_t cap = 0;
+  int64_t *counts = nullptr;        // host: S located counts (out)
+};
+
+// Draws the RNG values of S signals in signal order, runs the kernels on
+// the plan's stream and waits for them.  Returns 0 or -1 (set_error).
+int exec_batch(Plan *P, BatchIO &io);
+
+// ---------------------------------------------------------------- kernels
+namespace k {
+struct V12Launch {
+  int n, S, L, Lloc;
+  int B_loc, B_est, w_loc, w_est, num, W, comb_loops, thr;
+  const double2 *const *x;        // device array of S signal pointers
+  const int *params;              // S x (2L + comb_loops): a[L], ai[L], off[cl]
+  const double2 *taps_loc, *taps_est;
+  const float2 *taps_loc32, *taps_est32;
+  const double2 *win_loc, *win_est;
+  const double2 *tw64;
+  const float2 *tw32;
+  int tw_nmax;
+  bool fp32;       // estimation loops (and v2 location loops) in FP32
+  bool loc64;      // v1 under fp32: location loops stay FP64
+  bool ref_conj;
+  double2 *xs64;   // S x xs buckets of the FP64 loops
+  float2 *xs32;    // S x xs buckets of the FP32 loops
+  int64_t xs;
+  int *approved;   // S x comb_loops*num
+  int *num_comb;   // S
+  int *J;          // S x Lloc x num
+  int *hits;       // S x hits_cap
+  int64_t hits_cap;
+  int64_t *count;  // S located counts
+  int32_t *const *o_freq;   // S pointers or null
+  double2 *const *o_val;    // S pointers or null
+  double2 *const *o_dense;  // S pointers or null
+  int64_t o_cap;            // sparse capacity per signal
+  int64_t est_cap;          // upper bound of the located count per signal
+};
+int comb(const V12Launch &a, hipStream_t st);
+int fold_fft(const V12Launch &a, hipStream_t st);
+int select_vote_v1(const V12Launch &a, hipStream_t st);
+int estimate(const V12Launch &a, hipStream_t st);
+int zero_dense(double2 *out, int64_t n, hipStream_t st);
+}  // namespace k
+
+}  // namespace sfa

@@ -1,0 +1,537 @@
+// sFFT v1/v2 kernels for gfx950.
+//
+//   comb          v2 heavy-hitter prefilter: strided W-point sub-sample, FFT
+//                 in LDS, top-2k residues (src/computefourier-1.0-2.0.cc:48-82)
+//   comb_merge    sort+dedupe of several comb loops' residues (cf12 492-502)
+//   fold_fft      permuted-filter bucketization folded into B buckets, then
+//                 the B-point FFT in LDS (cf12 222-275)
+//   select        top-2k buckets of each location loop (cf12 278-302)
+//   vote_v1       exact-integer location votes (cf12 91-116) without an
+//                 n-sized score array
+//   estimate      per located frequency, the lower median over loops of
+//                 bucket / filter (cf12 340-419)
+//
+// Signals of a batch are independent; blockIdx encodes (signal, loop) or
+// (signal, chunk).  All index arithmetic is exact: for n = 2^p the low 32
+// bits of i * a already give (i * a) mod n.
+#include <algorithm>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace sfa {
+namespace k {
+
+using namespace sfa::dev;
+
+namespace {
+
+__device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
+
+__device__ __forceinline__ int64_t loop_offset(int l, int Lloc, int Bl, int Be)
+{
+  return (int64_t)min(l, Lloc) * Bl + (int64_t)max(0, l - Lloc) * Be;
+}
+
+// ---------------------------------------------------------------- comb
+// One block per (signal, comb loop).  LDS: W double2.
+__global__ void __launch_bounds__(kBlock) k_comb(V12Launch a)
+{
+  extern __shared__ double2 smem64[];
+  __shared__ int sh[256 + 16];
+  const int cl = a.comb_loops;
+  const int s = blockIdx.x / cl, c = blockIdx.x % cl;
+  const int stride = 2 * a.L + max(cl, 1);
+  const int off = a.params[(int64_t)s * stride + 2 * a.L + c];
+  const int W = a.W, logW = ilog2(W);
+  const int sigma = a.n / W;
+  const double2 *x = a.x[s];
+  for (int i = threadIdx.x; i < W; i += blockDim.x)
+    smem64[bitrev(i, logW)] = x[(int64_t)off + (int64_t)i * sigma];
+  __syncthreads();
+  lds_fft(smem64, logW, a.tw64, a.tw_nmax / W);
+  // |.|^2 into the same LDS, as doubles (staged through registers: W is at
+  // most kMaxLdsFft64 = 32 * kBlock)
+  constexpr int kPer = kMaxLdsFft64 / kBlock;
+  double v[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int i = threadIdx.x + q * kBlock;
+    if (i < W) {
+      const double2 z = smem64[i];
+      v[q] = z.x * z.x + z.y * z.y;
+    }
+  }
+  __syncthreads();
+  double *mag = reinterpret_cast<double *>(smem64);
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int i = threadIdx.x + q * kBlock;
+    if (i < W) mag[i] = v[q];
+  }
+  __syncthreads();
+  int *out = a.approved + ((int64_t)s * cl + c) * a.num;
+  block_top_num(mag, W, a.num, out, sh);
+  if (cl == 1 && threadIdx.x == 0) {
+    a.num_comb[s] = a.num;
+    a.count[s] = (int64_t)a.num * (a.n / W);
+  }
+}
+
+// Several comb loops: union of residues, ascending (bitmap over W in LDS).
+__global__ void __launch_bounds__(kBlock) k_comb_merge(V12Launch a)
+{
+  extern __shared__ unsigned bits[];
+  __shared__ int sh[16];
+  const int s = blockIdx.x;
+  const int W = a.W, words = (W + 31) / 32;
+  const int tot = a.comb_loops * a.num;
+  int *ap = a.approved + (int64_t)s * tot;
+  for (int i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int r = ap[i];
+    atomicOr(&bits[r >> 5], 1u << (r & 31));
+  }
+  __syncthreads();
+  const int C = (words + blockDim.x - 1) / blockDim.x;
+  const int lo = threadIdx.x * C, hi = min(words, lo + C);
+  int cnt = 0;
+  for (int wd = lo; wd < hi; wd++) cnt += __popc(bits[wd]);
+  int total;
+  int pos = block_exclusive_scan(cnt, sh, total);
+  // write after everyone has read the unmerged lists
+  __syncthreads();
+  for (int wd = lo; wd < hi; wd++) {
+    unsigned b = bits[wd];
+    while (b) {
+      const int bit = __ffs(b) - 1;
+      ap[pos++] = wd * 32 + bit;
+      b &= b - 1;
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.num_comb[s] = total;
+    a.count[s] = (int64_t)total * (a.n / W);
+  }
+}
+
+// ---------------------------------------------------------------- fold + FFT
+// One block per (signal, loop).  Each thread owns buckets m = tid + q*T and
+// sums, in ascending tap order like the reference, x[(i*ai) mod n]*tap[i]
+// over the taps i = m + r*B < w.  LDS: B double2 (or float2).
+constexpr int kQ = 8;  // buckets per thread in flight (independent gathers)
+
+__global__ void __launch_bounds__(kBlock) k_fold_fft64(V12Launch a, int l0, int nl)
+{
+  extern __shared__ double2 smem64[];
+  const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
+  const bool isloc = l < a.Lloc;
+  const int B = isloc ? a.B_loc : a.B_est;
+  const int w = isloc ? a.w_loc : a.w_est;
+  const double2 *__restrict__ taps = isloc ? a.taps_loc : a.taps_est;
+  const int stride = 2 * a.L + max(a.comb_loops, 1);
+  const unsigned ai = (unsigned)a.params[(int64_t)s * stride + a.L + l];
+  const double2 *__restrict__ x = a.x[s];
+  const unsigned mask = (unsigned)a.n - 1;
+  const int logB = ilog2(B);
+  const int T = blockDim.x;
+  const int R = (w + B - 1) / B;
+  for (int mb = threadIdx.x; mb < B; mb += T * kQ) {
+    double2 acc[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; q++) acc[q] = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; r++) {
+      double2 xv[kQ], tv[kQ];
+#pragma unroll
+      for (int q = 0; q < kQ; q++) {
+        const int m = mb + q * T;
+        const int i = m + r * B;
+        if (m < B && i < w) {
+          xv[q] = x[((unsigned)i * ai) & mask];
+          tv[q] = taps[i];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kQ; q++) {
+        const int m = mb + q * T;
+        const int i = m + r * B;
+        if (m < B && i < w) {
+          const double re = xv[q].x * tv[q].x - xv[q].y * tv[q].y;
+          const double im = xv[q].x * tv[q].y + xv[q].y * tv[q].x;
+          acc[q].x = acc[q].x + re;
+          acc[q].y = acc[q].y + im;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      const int m = mb + q * T;
+      if (m < B) smem64[bitrev(m, logB)] = acc[q];
+    }
+  }
+  __syncthreads();
+  lds_fft(smem64, logB, a.tw64, a.tw_nmax / B);
+  double2 *out = a.xs64 + (int64_t)s * a.xs +
+                 loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+  for (int j = threadIdx.x; j < B; j += T) out[j] = smem64[j];
+}
+
+// FP32 bucketization: products in FP32, compensated (2Sum) FP32 sums.
+__global__ void __launch_bounds__(kBlock) k_fold_fft32(V12Launch a, int l0, int nl)
+{
+  extern __shared__ float2 smem32[];
+  const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
+  const bool isloc = l < a.Lloc;
+  const int B = isloc ? a.B_loc : a.B_est;
+  const int w = isloc ? a.w_loc : a.w_est;
+  const float2 *__restrict__ taps = isloc ? a.taps_loc32 : a.taps_est32;
+  const int stride = 2 * a.L + max(a.comb_loops, 1);
+  const unsigned ai = (unsigned)a.params[(int64_t)s * stride + a.L + l];
+  const double2 *__restrict__ x = a.x[s];
+  const unsigned mask = (unsigned)a.n - 1;
+  const int logB = ilog2(B);
+  const int T = blockDim.x;
+  const int R = (w + B - 1) / B;
+  for (int mb = threadIdx.x; mb < B; mb += T * kQ) {
+    float2 sum[kQ], cmp[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      sum[q] = make_float2(0.f, 0.f);
+      cmp[q] = make_float2(0.f, 0.f);
+    }
+    for (int r = 0; r < R; r++) {
+      double2 xv[kQ];
+      float2 tv[kQ];
+#pragma unroll
+      for (int q = 0; q < kQ; q++) {
+        const int m = mb + q * T;
+        const int i = m + r * B;
+        if (m < B && i < w) {
+          xv[q] = x[((unsigned)i * ai) & mask];
+          tv[q] = taps[i];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kQ; q++) {
+        const int m = mb + q * T;
+        const int i = m + r * B;
+        if (m < B && i < w) {
+          const float2 xf = make_float2((float)xv[q].x, (float)xv[q].y);
+          const float2 p = cmul(xf, tv[q]);
+          // 2Sum: t = s + p exactly split into t and err
+          float t = sum[q].x + p.x, bp = t - sum[q].x;
+          cmp[q].x += (sum[q].x - (t - bp)) + (p.x - bp);
+          sum[q].x = t;
+          t = sum[q].y + p.y;
+          bp = t - sum[q].y;
+          cmp[q].y += (sum[q].y - (t - bp)) + (p.y - bp);
+          sum[q].y = t;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      const int m = mb + q * T;
+      if (m < B)
+        smem32[bitrev(m, logB)] =
+            make_float2(sum[q].x + cmp[q].x, sum[q].y + cmp[q].y);
+    }
+  }
+  __syncthreads();
+  lds_fft(smem32, logB, a.tw32, a.tw_nmax / B);
+  float2 *out = a.xs32 + (int64_t)s * a.xs +
+                loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+  for (int j = threadIdx.x; j < B; j += T) out[j] = smem32[j];
+}
+
+// ---------------------------------------------------------------- v1 votes
+// One block per (signal, location loop): top-num buckets by |.|^2.
+// LDS: B doubles.  v1 location loops are always FP64 (xs64): their top-2k
+// choice is made among leakage-level bucket powers FP32 cannot resolve.
+__global__ void __launch_bounds__(kBlock) k_select(V12Launch a)
+{
+  extern __shared__ double mag[];
+  __shared__ int sh[256 + 16];
+  const int s = blockIdx.x / a.Lloc, l = blockIdx.x % a.Lloc;
+  const int B = a.B_loc;
+  const int64_t base = (int64_t)s * a.xs + (int64_t)l * a.B_loc;
+  const double2 *xs = a.xs64 + base;
+  for (int i = threadIdx.x; i < B; i += blockDim.x) {
+    const double2 z = xs[i];
+    mag[i] = z.x * z.x + z.y * z.y;
+  }
+  __syncthreads();
+  block_top_num(mag, B, a.num, a.J + ((int64_t)s * a.Lloc + l) * a.num, sh);
+}
+
+// One block per signal.  A frequency f is a hit when, in at least thr of the
+// location loops, its permuted position p = f*ai mod n lies in the range
+// [ceil((J-1/2) n/B), ceil((J+1/2) n/B)) of a selected bucket J
+// (src/computefourier-1.0-2.0.cc:91-116).  Candidates come from the ranges
+// of loops 0..Lloc-thr; a candidate is counted only from the first such loop
+// that selects it.  LDS: Lloc * B bits of selected-bucket flags.
+__global__ void __launch_bounds__(kBlock) k_vote_v1(V12Launch a)
+{
+  extern __shared__ unsigned selbits[];
+  __shared__ int sh[16];
+  __shared__ int sh_base;
+  const int s = blockIdx.x;
+  const int B = a.B_loc, Lloc = a.Lloc, num = a.num, n = a.n;
+  const int words = (B + 31) / 32;
+  const int stride = 2 * a.L + max(a.comb_loops, 1);
+  const int *prm = a.params + (int64_t)s * stride;
+  const unsigned mask = (unsigned)n - 1;
+  const int m = n / B, half = m / 2, logm = ilog2(m);
+  for (int i = threadIdx.x; i < Lloc * words; i += blockDim.x) selbits[i] = 0;
+  __syncthreads();
+  const int *J = a.J + (int64_t)s * Lloc * num;
+  for (int i = threadIdx.x; i < Lloc * num; i += blockDim.x) {
+    const int l = i / num, b = J[i];
+    atomicOr(&selbits[l * words + (b >> 5)], 1u << (b & 31));
+  }
+  if (threadIdx.x == 0) sh_base = 0;
+  __syncthreads();
+  int *hits = a.hits + (int64_t)s * a.hits_cap;
+  const int enum_loops = Lloc - a.thr + 1;
+  const int64_t per_loop = (int64_t)num * m;
+  const int64_t total = (int64_t)enum_loops * per_loop;
+  for (int64_t c0 = 0; c0 < total; c0 += blockDim.x) {
+    const int64_t c = c0 + threadIdx.x;
+    int keep = 0;
+    unsigned loc = 0;
+    if (c < total) {
+      const int l0 = (int)(c / per_loop);
+      const int64_t r = c % per_loop;
+      const int jb = J[(int64_t)l0 * num + (int)(r >> logm)];
+      const int q = (int)(r & (m - 1));
+      const unsigned p = ((unsigned)jb * (unsigned)m - (unsigned)half +
+                          (unsigned)q + (unsigned)n) & mask;
+      loc = (p * (unsigned)prm[l0]) & mask;  // original frequency
+      int score = 0, earlier = 0;
+      for (int l = 0; l < Lloc; l++) {
+        const unsigned pl = (loc * (unsigned)prm[a.L + l]) & mask;
+        const unsigned bk = ((pl + (unsigned)half) & mask) >> logm;
+        const int sel = (selbits[l * words + (bk >> 5)] >> (bk & 31)) & 1;
+        score += sel;
+        earlier |= (l < l0) & sel;
+      }
+      keep = (score >= a.thr) && !earlier;
+    }
+    int tot;
+    const int pos = block_exclusive_scan(keep, sh, tot);
+    const int base = sh_base;
+    if (keep && base + pos < a.hits_cap) hits[base + pos] = (int)loc;
+    __syncthreads();
+    if (threadIdx.x == 0) sh_base = base + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.count[s] = min((int64_t)sh_base, a.hits_cap);
+}
+
+// ---------------------------------------------------------------- estimate
+// One thread per located frequency h: for every loop, the bucket h hashes
+// to, divided by the filter response at h's offset from the bucket centre
+// (ties of the half-bucket round down, cf12 371-378); then the lower median
+// (rank (L-1)/2) of real and imaginary parts separately (cf12 406-413).
+template <int LMAX>
+__device__ __forceinline__ void sort_net(double (&v)[LMAX])
+{
+  // bitonic sorting network, ascending
+#pragma unroll
+  for (int k2 = 2; k2 <= LMAX; k2 <<= 1) {
+#pragma unroll
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < LMAX; i++) {
+        const int p = i ^ j;
+        if (p > i) {
+          const bool up = (i & k2) == 0;
+          const double lo = fmin(v[i], v[p]), hi = fmax(v[i], v[p]);
+          v[i] = up ? lo : hi;
+          v[p] = up ? hi : lo;
+        }
+      }
+    }
+  }
+}
+
+template <int LMAX, bool FP32>
+__global__ void __launch_bounds__(kBlock) k_estimate(V12Launch a)
+{
+  __shared__ unsigned s_ai[kMaxLoops];
+  const int64_t chunks = (a.est_cap + blockDim.x - 1) / blockDim.x;
+  const int s = (int)(blockIdx.x / chunks);
+  const int64_t t = (blockIdx.x % chunks) * blockDim.x + threadIdx.x;
+  const int stride = 2 * a.L + max(a.comb_loops, 1);
+  if (threadIdx.x < a.L)
+    s_ai[threadIdx.x] = (unsigned)a.params[(int64_t)s * stride + a.L + threadIdx.x];
+  __syncthreads();
+  const int64_t cnt = a.count[s];
+  if (t >= cnt) return;
+  const int n = a.n;
+  const unsigned mask = (unsigned)n - 1;
+  unsigned h;
+  if (a.comb_loops > 0) {
+    const int nc = a.num_comb[s];
+    const int64_t j = t / nc;
+    const int i = (int)(t % nc);
+    h = (unsigned)(j * a.W + a.approved[(int64_t)s * a.comb_loops * a.num + i]);
+  } else {
+    h = (unsigned)a.hits[(int64_t)s * a.hits_cap + t];
+  }
+  double vr[LMAX], vi[LMAX];
+#pragma unroll
+  for (int l = 0; l < LMAX; l++) {
+    if (l < a.L) {
+      const bool isloc = l < a.Lloc;
+      const int B = isloc ? a.B_loc : a.B_est;
+      const int m = n / B;
+      const unsigned p = (s_ai[l] * h) & mask;
+      int hashed = (int)(p / (unsigned)m);
+      int dist = (int)(p % (unsigned)m);
+      if (dist > m / 2) {
+        hashed = (hashed + 1) & (B - 1);
+        dist -= m;
+      }
+      const double2 G = (isloc ? a.win_loc : a.win_est)[dist + m / 2];
+      const int64_t idx = (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est) + hashed;
+      double2 U;
+      if (FP32 && !(a.loc64 && isloc)) {
+        const float2 u = a.xs32[idx];
+        U = make_double2((double)u.x, (double)u.y);
+      } else {
+        U = a.xs64[idx];
+      }
+      const double den = G.x * G.x + G.y * G.y;
+      vr[l] = (U.x * G.x + U.y * G.y) / den;
+      vi[l] = a.ref_conj ? (U.x * G.y - U.y * G.x) / den
+                         : (U.y * G.x - U.x * G.y) / den;
+    } else {
+      vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
+      vi[l] = vr[l];
+    }
+  }
+  sort_net<LMAX>(vr);
+  sort_net<LMAX>(vi);
+  const int rank = (a.L - 1) / 2;
+  double mr = vr[0], mi = vi[0];
+#pragma unroll
+  for (int l = 1; l < LMAX; l++) {
+    if (l == rank) {
+      mr = vr[l];
+      mi = vi[l];
+    }
+  }
+  const double2 val = make_double2(mr, mi);
+  if (a.o_freq && t < a.o_cap) {
+    int32_t *of = a.o_freq[s];
+    double2 *ov = a.o_val[s];
+    if (of) of[t] = (int32_t)h;
+    if (ov) ov[t] = val;
+  }
+  if (a.o_dense) {
+    double2 *od = a.o_dense[s];
+    if (od) od[h] = val;
+  }
+}
+
+__global__ void k_zero(double2 *out, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = i; j < n; j += stride) out[j] = make_double2(0.0, 0.0);
+}
+
+// Kernels whose dynamic LDS can exceed the default 64 KiB opt in to the
+// gfx950 maximum once.
+template <typename K>
+void allow_lds(K kernel, size_t bytes)
+{
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+}
+
+int launched(const char *what)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int comb(const V12Launch &a, hipStream_t st)
+{
+  const size_t lds = (size_t)a.W * sizeof(double2);
+  allow_lds(k_comb, lds);
+  hipLaunchKernelGGL(k_comb, dim3(a.S * a.comb_loops), dim3(kBlock), lds, st, a);
+  if (launched("comb") != 0) return -1;
+  if (a.comb_loops > 1) {
+    const size_t lb = (size_t)((a.W + 31) / 32) * 4;
+    hipLaunchKernelGGL(k_comb_merge, dim3(a.S), dim3(kBlock), lb, st, a);
+    if (launched("comb_merge") != 0) return -1;
+  }
+  return 0;
+}
+
+int fold_fft(const V12Launch &a, hipStream_t st)
+{
+  // loops [0, n64) in FP64, [n64, L) in FP32
+  const int n64 = !a.fp32 ? a.L : (a.loc64 ? a.Lloc : 0);
+  const int B64 = n64 == 0 ? 0 : (n64 > a.Lloc ? max(a.B_loc, a.B_est) : a.B_loc);
+  const int B32 = n64 >= a.L ? 0 : (n64 == 0 ? max(a.B_loc, a.B_est) : a.B_est);
+  if (n64 > 0) {
+    allow_lds(k_fold_fft64, (size_t)B64 * sizeof(double2));
+    hipLaunchKernelGGL(k_fold_fft64, dim3(a.S * n64), dim3(kBlock),
+                       (size_t)B64 * sizeof(double2), st, a, 0, n64);
+    if (launched("fold_fft64") != 0) return -1;
+  }
+  if (n64 < a.L) {
+    allow_lds(k_fold_fft32, (size_t)B32 * sizeof(float2));
+    hipLaunchKernelGGL(k_fold_fft32, dim3(a.S * (a.L - n64)), dim3(kBlock),
+                       (size_t)B32 * sizeof(float2), st, a, n64, a.L - n64);
+    if (launched("fold_fft32") != 0) return -1;
+  }
+  return 0;
+}
+
+int select_vote_v1(const V12Launch &a, hipStream_t st)
+{
+  allow_lds(k_select, (size_t)a.B_loc * sizeof(double));
+  hipLaunchKernelGGL(k_select, dim3(a.S * a.Lloc), dim3(kBlock),
+                     (size_t)a.B_loc * sizeof(double), st, a);
+  if (launched("select") != 0) return -1;
+  const size_t lb = (size_t)a.Lloc * ((a.B_loc + 31) / 32) * 4;
+  hipLaunchKernelGGL(k_vote_v1, dim3(a.S), dim3(kBlock), lb, st, a);
+  return launched("vote_v1");
+}
+
+int estimate(const V12Launch &a, hipStream_t st)
+{
+  const int64_t chunks = (a.est_cap + kBlock - 1) / kBlock;
+  const dim3 grid((unsigned)(chunks * a.S));
+  if (a.L <= 16) {
+    if (a.fp32) hipLaunchKernelGGL((k_estimate<16, true>), grid, dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL((k_estimate<16, false>), grid, dim3(kBlock), 0, st, a);
+  } else {
+    if (a.fp32) hipLaunchKernelGGL((k_estimate<32, true>), grid, dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL((k_estimate<32, false>), grid, dim3(kBlock), 0, st, a);
+  }
+  return launched("estimate");
+}
+
+int zero_dense(double2 *out, int64_t n, hipStream_t st)
+{
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, out, n);
+  return launched("zero_dense");
+}
+
+}  // namespace k
+}  // namespace sfa

@@ -1,0 +1,250 @@
+"""Python host interface over libsfft_amd, mirroring the reference's ctypes
+class (python/sfft/sfft.py:33-85): same constructor arguments, validation
+and ``execute(a)`` returning the dense spectrum estimate.
+
+GPU-resident use goes through ``execute_device`` / ``execute_many_sparse``
+with torch tensors (torch only provides device memory and the stream).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+# flags copied from fftw3.h (python/sfft/sfft.py:6-8)
+FFTW_MEASURE = 0
+FFTW_ESTIMATE = 1 << 6
+
+# python/sfft/sfft.py:10-31: the (n, k) pairs the reference's wrapper allows
+# for v1/v2.
+V1_V2_INPUT_PARAMETERS = [
+    {"size": 8192, "sparsity": 50}, {"size": 16384, "sparsity": 50},
+    {"size": 32768, "sparsity": 50}, {"size": 65536, "sparsity": 50},
+    {"size": 131072, "sparsity": 50}, {"size": 262144, "sparsity": 50},
+    {"size": 524288, "sparsity": 50}, {"size": 1048576, "sparsity": 50},
+    {"size": 2097152, "sparsity": 50}, {"size": 4194304, "sparsity": 50},
+    {"size": 8388608, "sparsity": 50}, {"size": 16777216, "sparsity": 50},
+    {"size": 4194304, "sparsity": 50}, {"size": 4194304, "sparsity": 100},
+    {"size": 4194304, "sparsity": 200}, {"size": 4194304, "sparsity": 500},
+    {"size": 4194304, "sparsity": 1000}, {"size": 4194304, "sparsity": 2000},
+    {"size": 4194304, "sparsity": 2500}, {"size": 4194304, "sparsity": 4000},
+]
+
+PRECISIONS = {"fp64": _lib.SFFT_PRECISION_FP64,
+              "fp32_refine": _lib.SFFT_PRECISION_FP32_REFINE}
+
+
+class SfftError(RuntimeError):
+    pass
+
+
+def derive_params(n, k, version):
+    """Parameters libsfft_amd derives for (n, k, version 1..3), host only.
+
+    Returns (ok, dict); ok is False when sfft_make_plan would reject it.
+    """
+    p = (C.c_int * 10)()
+    r = _lib.load().sfft_derive_params(n, k, version - 1, p)
+    if r < 0:
+        raise ValueError(_lib.last_error())
+    if version == 3:
+        keys = ["W_Man", "B_g1", "B_g2", "w_g1", "w_g2"]
+    else:
+        keys = ["B_loc", "B_est", "B_thresh", "W_comb", "comb_loops",
+                "loc_loops", "est_loops", "thr_loops", "w_loc", "w_est"]
+    d = {kk: p[i] for i, kk in enumerate(keys)}
+    return r == 0, d
+
+
+def design_filter(lobefrac, tolerance, n, b):
+    """The library's host filter design: (time taps, n-point response)."""
+    w_cap = int((1 / np.pi) * (1 / lobefrac) * np.arccosh(1. / tolerance)) + 2
+    t = np.zeros(w_cap, np.complex128)
+    f = np.zeros(n, np.complex128)
+    w = _lib.load().sfft_design_filter(lobefrac, tolerance, n, b,
+                                       t.ctypes.data_as(C.c_void_p), w_cap,
+                                       f.ctypes.data_as(C.c_void_p))
+    if w < 0:
+        raise SfftError(_lib.last_error())
+    return t[:w].copy(), f
+
+
+class sfft:
+    """sFFT plan (python/sfft/sfft.py:33-85).
+
+    ``strict`` keeps the reference wrapper's (n, k) whitelist for v1/v2;
+    with ``strict=False`` every (n, k) the library accepts is allowed.
+    """
+
+    def __init__(self, length=16384, sparsity=50, version=1,
+                 optimization=FFTW_ESTIMATE, strict=True, precision="fp64",
+                 seed=None):
+        for name, v in (("length", length), ("sparsity", sparsity),
+                        ("version", version), ("optimization", optimization)):
+            if not isinstance(v, (int, np.integer)):
+                raise TypeError("%s is not an integer" % name)
+        if version not in [1, 2, 3]:
+            raise ValueError("sFFT version %d is not valid.  Try 1, 2, or 3." % (version))
+        if strict and version in [1, 2] and \
+                {"size": length, "sparsity": sparsity} not in V1_V2_INPUT_PARAMETERS:
+            raise ValueError("n = %d and k = %d is not a valid input parameter combination "
+                             "for sFFT version %d." % (length, sparsity, version))
+        if optimization not in [FFTW_MEASURE, FFTW_ESTIMATE]:
+            raise ValueError("FFTW optimization %d is not valid." % (optimization))
+        self.length = int(length)
+        self.sparsity = int(sparsity)
+        self.version = int(version)
+        self.optimization = optimization
+        self._L = _lib.load()
+        self.sfft_plan = self._L.sfft_make_plan(self.length, self.sparsity,
+                                                self.version - 1, FFTW_ESTIMATE)
+        if not self.sfft_plan:
+            self.sfft_plan = None
+            raise SfftError("sfft_make_plan(%d, %d, v%d) failed: %s"
+                            % (length, sparsity, version, _lib.last_error()))
+        self.set_precision(precision)
+        if seed is not None:
+            self.seed(*seed)
+        self._stream = None
+
+    # ------------------------------------------------------------ reference
+    def execute(self, a):
+        """Dense spectrum estimate of one host signal (numpy in, numpy out)."""
+        a = np.require(np.asanyarray(a), np.complex128, ['C', 'ALIGNED'])
+        if a.shape != (self.length,):
+            raise ValueError("input must have shape (%d,)" % self.length)
+        b = np.empty_like(a)
+        self._L.sfft_exec(self.sfft_plan, a.ctypes.data_as(C.c_void_p),
+                          b.ctypes.data_as(C.c_void_p))
+        self._check()
+        return b
+
+    def __del__(self):
+        if getattr(self, "sfft_plan", None):
+            self._L.sfft_free_plan(self.sfft_plan)
+            self.sfft_plan = None
+
+    # ------------------------------------------------------------ extensions
+    def _check(self):
+        e = _lib.last_error()
+        if e:
+            raise SfftError(e)
+
+    def set_precision(self, precision):
+        code = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        if self._L.sfft_plan_set_precision(self.sfft_plan, code) != 0:
+            raise SfftError(_lib.last_error())
+        self.precision = precision
+
+    def seed(self, random_seed=17, drand48_seed=4242):
+        """Private RNG streams equal to srandom(random_seed)/srand48(..)."""
+        self._L.sfft_plan_seed(self.sfft_plan, random_seed, drand48_seed)
+
+    def set_reference_conjugate(self, enabled):
+        self._L.sfft_plan_set_reference_conjugate(self.sfft_plan, int(bool(enabled)))
+
+    def params(self):
+        p = (C.c_int * 10)()
+        self._L.sfft_plan_params(self.sfft_plan, p)
+        return list(p)
+
+    @property
+    def capacity(self):
+        return int(self._L.sfft_plan_output_capacity(self.sfft_plan))
+
+    def memory(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._L.sfft_plan_memory(self.sfft_plan, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def _bind_stream(self, torch):
+        s = torch.cuda.current_stream().cuda_stream
+        if s != self._stream:
+            self._L.sfft_plan_set_stream(self.sfft_plan, C.c_void_p(s))
+            self._stream = s
+
+    def execute_device(self, x, out):
+        """sfft_exec on device tensors: x, out complex128 of length n."""
+        import torch
+        assert x.is_cuda and out.is_cuda and x.dtype == torch.complex128
+        assert out.dtype == torch.complex128 and x.numel() == out.numel() == self.length
+        self._bind_stream(torch)
+        self._L.sfft_exec(self.sfft_plan, C.c_void_p(x.data_ptr()),
+                          C.c_void_p(out.data_ptr()))
+        self._check()
+        return out
+
+    def execute_many_device(self, xs, outs):
+        """sfft_exec_many on lists of device tensors (dense outputs)."""
+        import torch
+        self._bind_stream(torch)
+        num = len(xs)
+        ins = (C.c_void_p * num)(*[t.data_ptr() for t in xs])
+        ous = (C.c_void_p * num)(*[t.data_ptr() for t in outs])
+        self._L.sfft_exec_many(self.sfft_plan, num, ins, ous)
+        self._check()
+        return outs
+
+    def make_sparse_outputs(self, num, device="cuda"):
+        """Device buffers for execute_many_sparse: (freq[num,cap], val[num,cap])."""
+        import torch
+        cap = self.capacity
+        freq = torch.empty((num, cap), dtype=torch.int32, device=device)
+        val = torch.empty((num, cap), dtype=torch.complex128, device=device)
+        return freq, val
+
+    def execute_many_sparse(self, xs, freq, val):
+        """sfft_exec_many_sparse: xs list of device signals (or a 2-D tensor),
+        results into freq/val [num, cap]; returns the located counts."""
+        import torch
+        self._bind_stream(torch)
+        num = len(xs)
+        ptrs = [t.data_ptr() for t in xs] if isinstance(xs, (list, tuple)) \
+            else [xs[i].data_ptr() for i in range(num)]
+        ins = (C.c_void_p * num)(*ptrs)
+        outs = (_lib.SfftOutput * num)()
+        cap = freq.shape[1]
+        for i in range(num):
+            outs[i].capacity = cap
+            outs[i].freq = freq[i].data_ptr()
+            outs[i].val = val[i].data_ptr()
+        r = self._L.sfft_exec_many_sparse(self.sfft_plan, num, ins, outs)
+        if r != 0:
+            raise SfftError(_lib.last_error())
+        return np.array([outs[i].count for i in range(num)], dtype=np.int64)
+
+    STAGES = ["comb", "fold_fft", "select_vote", "estimate", "v3_loop"]
+
+    def enable_timing(self, enabled=True):
+        self._L.sfft_plan_enable_timing(self.sfft_plan, int(bool(enabled)))
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} from the library's HIP events."""
+        ms = (C.c_double * 5)()
+        ln = (C.c_int64 * 5)()
+        self._L.sfft_plan_stage_times(self.sfft_plan, ms, ln)
+        return {s: (ms[i], ln[i]) for i, s in enumerate(self.STAGES)}
+
+    def debug_buckets(self, signal=0, count=1 << 30):
+        """Bucket values (after the bucket FFTs) of the last v1/v2 batch."""
+        cnt = min(count, 1 << 26)
+        buf = np.zeros(cnt, np.complex128)
+        m = self._L.sfft_debug_buckets(self.sfft_plan, signal,
+                                       buf.ctypes.data_as(C.c_void_p), cnt)
+        if m < 0:
+            raise SfftError(_lib.last_error())
+        return buf[:m].copy()
+
+    def execute_sparse_host(self, x):
+        """One host signal, sparse result on the host: (freqs, values)."""
+        x = np.require(np.asanyarray(x), np.complex128, ['C', 'ALIGNED'])
+        cap = self.capacity
+        fr = np.zeros(cap, np.int32)
+        va = np.zeros(cap, np.complex128)
+        o = _lib.SfftOutput(0, cap, fr.ctypes.data, va.ctypes.data)
+        r = self._L.sfft_exec_sparse(self.sfft_plan, x.ctypes.data_as(C.c_void_p),
+                                     C.byref(o))
+        if r != 0:
+            raise SfftError(_lib.last_error())
+        m = min(o.count, cap)
+        return fr[:m].copy(), va[:m].copy()

@@ -1,0 +1,287 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each check is independent of the oracle's own code: the reference's own
+helpers compiled from its sources (oracle/_ref), numpy/scipy, closed forms,
+invariants, or brute force on tiny inputs.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.signal
+
+import oracle
+from sfft_amd.signals import signal_np
+
+REF = oracle.ref_helpers()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference tree)")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- _ref pins
+@needs_ref
+def test_gcd_mod_inverse_floor_pow2_vs_reference():
+    rng = np.random.default_rng(0)
+    L = oracle.lib()
+    for n in [8192, 1 << 16, 1 << 20, 1 << 22, 1 << 24]:
+        for a in list(rng.integers(1, n, 300)) + [1, n - 1]:
+            a = int(a)
+            assert L.or_gcd(a, n) == REF.ref_gcd(a, n)
+            if a % 2 == 1:
+                inv = L.or_mod_inverse(a, n)
+                assert inv == REF.ref_mod_inverse(a, n)
+                assert (inv * a) % n == 1
+    assert L.or_gcd(0, 1024) == REF.ref_gcd(0, 1024) == 1024
+    for x in [0.0, 0.5, 1.0, 1.5, 2.0, 3.0, 1023.9, 1024.0, 3493.0, 1e7]:
+        assert L.or_floor_to_pow2(x) == REF.ref_floor_to_pow2(x)
+
+
+@needs_ref
+@pytest.mark.parametrize("n,num", [(16, 3), (2048, 128), (4096, 128), (512, 100), (101, 100)])
+def test_find_largest_indices_vs_reference_with_ties(n, num):
+    rng = np.random.default_rng(n + num)
+    for trial in range(20):
+        if trial % 3 == 0:   # heavy ties around the cutoff
+            s = rng.integers(0, 4, n).astype(np.float64)
+        elif trial % 3 == 1:
+            s = np.round(rng.exponential(size=n), 2)
+        else:
+            s = rng.exponential(size=n)
+        got = oracle.find_largest_indices(s, num)
+        ref = np.zeros(num, np.int32)
+        tmp = np.zeros(n, np.float64)
+        s2 = s.copy()
+        REF.ref_find_largest_indices(_p(ref), num, _p(s2), n, _p(tmp))
+        assert np.array_equal(got, ref)
+
+
+@needs_ref
+def test_radix_sort_vs_reference():
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 1 << 24, 5000).astype(np.int32)
+    r = a.copy()
+    REF.ref_radix_sort(_p(r), len(r))
+    o = a.copy()
+    oracle.lib().or_sort_ints(_p(o), len(o))
+    assert np.array_equal(o, r)
+
+
+@needs_ref
+@pytest.mark.parametrize("with_comb", [0, 1])
+def test_parameter_tables_vs_reference(with_comb):
+    L = oracle.lib()
+    keys_n = [1 << p for p in range(13, 25)] + [12345]
+    keys_k = [50, 100, 200, 500, 1000, 2000, 2500, 4000, 64]
+    for table_k, keys in ((0, keys_n), (1, keys_k)):
+        for key in keys:
+            d_ref = np.array([-1.0] * 5)
+            i_ref = np.array([-1] * 4, dtype=np.int32)
+            REF.ref_get_params(table_k, key, with_comb, _p(d_ref), _p(i_ref))
+            d = [C.c_double(-1.0) for _ in range(5)]
+            i = [C.c_int(-1) for _ in range(4)]
+            fn = L.or_get_K_params if table_k else L.or_get_N_params
+            fn.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 9
+            fn(key, with_comb, C.byref(d[0]), C.byref(d[1]), C.byref(d[2]), C.byref(i[0]),
+               C.byref(i[1]), C.byref(i[2]), C.byref(i[3]), C.byref(d[3]), C.byref(d[4]))
+            assert [x.value for x in d] == list(d_ref), (table_k, key)
+            assert [x.value for x in i] == list(i_ref), (table_k, key)
+
+
+# ---------------------------------------------------------------- derived sizes
+@pytest.mark.parametrize("v,n,k,expect", [
+    # (version, n, k): B_loc, B_est, B_thresh, W_comb, loops, thr, w_loc, w_est
+    (1, 1 << 20, 50, (2048, 512, 100, 512, 4, 12, 3, 29907, 9843)),
+    (2, 1 << 22, 100, (4096, 4096, 200, 2048, 4, 16, 3, 53125, 53125)),
+    (2, 1 << 22, 64, (2048, 2048, 128, 4096, 4, 16, 3, 42503, 42503)),
+    (2, 1 << 20, 50, (512, 512, 100, 16384, 4, 12, 2, 7471, 9843)),
+])
+def test_derived_plan_sizes(v, n, k, expect):
+    """Plan sizes recomputed by hand from src/sfft.cc:298-372."""
+    p = oracle.v12_params(n, k, v == 2)
+    got = (p.B_loc, p.B_est, p.B_thresh, p.W_comb, p.loc_loops, p.est_loops,
+           p.thr_loops, p.w_loc, p.w_est)
+    assert got == expect
+
+
+def test_k_above_50_uses_defaults():
+    """The reference looks the K table up with n (src/sfft.cc:316-321)."""
+    for k in [51, 64, 100, 1000]:
+        p = oracle.v12_params(1 << 22, k, True)
+        assert (p.Bcst_loc, p.Bcst_est, p.Comb_cst, p.loc_loops, p.est_loops,
+                p.thr_loops, p.comb_loops) == (1, 1, 2, 4, 16, 3, 1)
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 1000, 1024, 4096, 12167, 42503, 1 << 16])
+def test_fft_vs_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    f = np.fft.fft(x)
+    assert np.abs(oracle.fft(x) - f).max() <= 1e-13 * max(1.0, np.abs(f).max())
+    b = np.fft.ifft(x) * n
+    assert np.abs(oracle.fft(x, backward=True) - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+
+
+# ---------------------------------------------------------------- filters
+@pytest.mark.parametrize("bb,tol", [(3493, 1e-8), (809, 1e-8), (3238, 1e-6), (64, 1e-8)])
+def test_dolph_chebyshev_vs_scipy(bb, tol):
+    """src/filters.cc:70-86 is the textbook Dolph-Chebyshev window."""
+    lobefrac = 0.5 / bb
+    x = oracle.dolph_chebyshev(lobefrac, tol)
+    w = len(x)
+    assert w % 2 == 1
+    assert np.abs(x.imag).max() == 0.0
+    ref = scipy.signal.windows.chebwin(w, at=-20 * np.log10(tol))
+    a = x.real / x.real.max()
+    assert np.abs(a - ref).max() < 1e-6
+
+
+def test_multiple_filter_invariants():
+    n, bb, tol = 1 << 16, 200, 1e-8
+    b = int(1.2 * 1.1 * n / bb)
+    x = oracle.dolph_chebyshev(0.5 / bb, tol)
+    w = len(x)
+    time, freq = oracle.make_multiple(x, n, b)
+    # normalised: max |freq| = 1
+    assert abs(np.abs(freq).max() - 1) < 1e-12
+    # the time taps are the inverse DFT of freq, truncated to w: the DFT of
+    # the zero-padded taps equals freq up to the truncation leakage
+    pad = np.zeros(n, np.complex128)
+    pad[:w] = time
+    G = np.fft.fft(pad)
+    assert np.abs(G - freq).max() < 50 * tol
+    # pass band flat around 0 (|f| < b/6; the window's main lobe spans
+    # ~2n/BB bins of transition), stop band at ~tol beyond b
+    f = np.fft.fftfreq(n, 1.0 / n).astype(int)
+    mag = np.abs(freq)
+    assert np.all(np.abs(mag[np.abs(f) < b // 6] - 1) < 1e-5)
+    assert mag[np.abs(f) > b].max() < 10 * tol
+    # |freq| is symmetric: about bin 0 for odd b, about bin +1/2 for even b
+    # (the running sum of b bins is re-centred by b/2)
+    j = np.arange(1, n // 2)
+    mirror = (n - j) % n if b % 2 else (n + 1 - j) % n
+    assert np.abs(mag[j] - mag[mirror]).max() < 1e-9
+
+
+# ---------------------------------------------------------------- v1/v2
+def _run(version, n, k, x, seed=(17, 4242)):
+    oracle.seed_glibc(*seed)
+    P = oracle.V12(n, k, version == 2)
+    a, ai, off = P.draw()
+    return P, a, ai, off, P.exec(x, a, ai, off, trace=True)
+
+
+def test_glibc_rng_draws_are_odd_and_invertible():
+    oracle.seed_glibc(17, 4242)
+    P = oracle.V12(1 << 16, 50, True)
+    a, ai, off = P.draw()
+    assert np.all(a % 2 == 1)
+    assert np.all((a.astype(np.int64) * ai) % (1 << 16) == 1)
+    assert np.all((0 <= off) & (off < (1 << 16) // P.p.W_comb))
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_buckets_single_tone_closed_form(version):
+    """For one tone X at f0, bucket J of loop j is exactly
+    X * G_j[(J n/B - ai_j f0) mod n] with G_j the n-point DFT of the taps."""
+    n, k = 1 << 14, 50
+    f0, X = 1234, np.exp(0.7j)
+    x = X * np.exp(2j * np.pi * f0 * np.arange(n) / n)
+    P, a, ai, off, r = _run(version, n, k, x)
+    filt = P.filters()
+    p = P.p
+    off_l = 0
+    for j in range(P.loops):
+        loc = j < p.loc_loops
+        B = p.B_loc if loc else p.B_est
+        t = filt["loc_time" if loc else "est_time"]
+        pad = np.zeros(n, np.complex128)
+        pad[:len(t)] = t
+        G = np.fft.fft(pad)
+        m = n // B
+        idx = (np.arange(B) * m - int(ai[j]) * f0) % n
+        expect = X * G[idx]
+        got = r["x_samp"][off_l:off_l + B]
+        assert np.abs(got - expect).max() < 1e-12
+        off_l += B
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_single_tone_value_is_conjugate(version):
+    """estimate_values returns conj(bucket/G) (cf12 383-398): for one tone the
+    located value is conj(X) up to the filter's truncation leakage."""
+    n, k = 1 << 16, 50
+    f0, X = 4321, np.exp(2.1j)
+    x = X * np.exp(2j * np.pi * f0 * np.arange(n) / n)
+    _, _, _, _, r = _run(version, n, k, x)
+    assert abs(r["out"][f0] - np.conj(X)) < 1e-6
+    assert abs(r["out"][f0] - X) > 1.0
+
+
+def test_v2_located_set_is_comb_prefill():
+    """cf12 505-512 + 137-180: every appended hit is a prefill alias, so the
+    support of out is {j W + c}; c = top-2k residues of |FFT_W(x[off + i n/W])|^2
+    computed here with numpy."""
+    n, k = 1 << 16, 50
+    x, f, v = signal_np(n, k, seed=9, snr_db=30.0)
+    P, a, ai, off, r = _run(2, n, k, x)
+    W, num = P.p.W_comb, P.p.B_thresh
+    approved = set()
+    for c in range(P.p.comb_loops):
+        sig = n // W
+        s = np.fft.fft(x[off[c] + sig * np.arange(W)])
+        mag = s.real * s.real + s.imag * s.imag
+        approved |= set(np.argsort(-mag, kind="stable")[:num].tolist())
+    expect = np.sort(np.array([j * W + c for j in range(n // W) for c in approved]))
+    assert np.array_equal(np.nonzero(r["out"])[0], expect)
+    assert set(np.unique(r["hits"]).tolist()) == set(expect.tolist())
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_noiseless_recovery(version):
+    n, k = 1 << 17, 50
+    x, f, v = signal_np(n, k, seed=21)
+    _, _, _, _, r = _run(version, n, k, x)
+    out = r["out"]
+    assert np.all(out[f] != 0)
+    assert np.abs(out[f] - np.conj(v)).max() < 1e-6
+
+
+def test_v1_votes_brute_force():
+    """Hits of v1 = frequencies whose permuted position falls in a selected
+    bucket's range in >= thr location loops (brute force over all n)."""
+    n, k = 1 << 14, 50
+    x, f, v = signal_np(n, k, seed=4, snr_db=20.0)
+    P, a, ai, off, r = _run(1, n, k, x)
+    p = P.p
+    B, m = p.B_loc, n // p.B_loc
+    score = np.zeros(n, np.int64)
+    allf = np.arange(n, dtype=np.int64)
+    for j in range(p.loc_loops):
+        pos = (allf * int(ai[j])) % n
+        bucket = ((pos + m // 2) % n) // m
+        sel = np.zeros(B, bool)
+        sel[r["J"][j]] = True
+        score += sel[bucket]
+    expect = np.nonzero(score >= p.thr_loops)[0]
+    assert np.array_equal(np.sort(r["hits"]), expect)
+
+
+# ---------------------------------------------------------------- v3
+@pytest.mark.parametrize("n,k", [(1 << 16, 50), (1 << 18, 100), (1 << 20, 200)])
+def test_v3_noiseless_exact_support(n, k):
+    x, f, v = signal_np(n, k, seed=n + k)
+    oracle.seed_glibc(17, 4242)
+    P = oracle.V3(n, k)
+    r = P.exec(x, P.draw())
+    if r["early"]:
+        pytest.skip("first Mansour pass found all k: reference leaves out all zero")
+    loc = np.nonzero(r["out"])[0]
+    # every tone is found; the peel loop may leave residual-level extras
+    assert np.all(np.isin(f, loc))
+    extra = np.setdiff1d(loc, f)
+    assert np.abs(r["out"][extra]).max(initial=0.0) < 1e-4
+    assert np.abs(r["out"][f] - v).max() < 1e-4
