@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-fp32", action="store_true",
                     help="time only the headline form (for profilers)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first copy 1 GiB device-to-device (a kernel of known traffic "
+                         "that scales the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -75,6 +78,12 @@ def main():
     n, k, S = args.n, args.k, args.batch
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
     seeds = list(range(1000, 1000 + S))
     x, fs, vs = signals_device(n, k, seeds, snr_db=args.snr, device=dev)
     xs = [x[i] for i in range(S)]

@@ -1125,11 +1125,11 @@ long or_v3_exec(const or_v3_plan *P, const or_cplx *x, const int *d,
     free(s0);
     free(s1);
     fft_interleaved2(GS, B1);
-    int *ord = omap_sorted_order(&ans);
+    /* cf3 881-887 iterates a tr1::unordered_map, whose order no rebuild can
+     * reproduce; the map is iterated in insertion order here (and on the
+     * GPU), which changes only the rounding of these subtractions. */
     for (long q = 0; q < ans.size; q++)
-      update_gaussian(ans.keys[ord[q]], ans.vals[ord[q]], GS, P->freq1, n, B1,
-                      gG, 1);
-    free(ord);
+      update_gaussian(ans.keys[q], ans.vals[q], GS, P->freq1, n, B1, gG, 1);
     fn = est_gauss(B1, GS, gG, n, 1, 0, 1, P->freq1, EF, EV);
     for (int i = 0; i < fn; i++) {
       long e = omap_at(&ans, EF[i]);
@@ -1159,13 +1159,9 @@ long or_v3_exec(const or_v3_plan *P, const or_cplx *x, const int *d,
     gauss_fold(GP, B2, w2, P->time2, perm, perm + 1);
     free(perm);
     fft_interleaved2(GP, B2);
-    int *ord = omap_sorted_order(&ans);
-    for (long q = 0; q < ans.size; q++) {
-      int key = ans.keys[ord[q]];
-      update_gaussian((timesmod(key, ai, n) + shift) % n, ans.vals[ord[q]], GP,
+    for (long q = 0; q < ans.size; q++) /* insertion order, see above */
+      update_gaussian((timesmod(ans.keys[q], ai, n) + shift) % n, ans.vals[q], GP,
                       P->freq2, n, B2, wmul(a, gG), 1);
-    }
-    free(ord);
     fn = est_gauss(B2, GP, gG, n, a, b, 1, P->freq2, EF, EV);
     for (int i = 0; i < fn; i++) {
       long e = omap_at(&ans, EF[i]);

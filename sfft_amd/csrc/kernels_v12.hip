@@ -330,91 +330,76 @@ __global__ void __launch_bounds__(kBlock) k_vote_v1(V12Launch a)
 }
 
 // ---------------------------------------------------------------- estimate
-// One thread per located frequency h: for every loop, the bucket h hashes
-// to, divided by the filter response at h's offset from the bucket centre
-// (ties of the half-bucket round down, cf12 371-378); then the lower median
-// (rank (L-1)/2) of real and imaginary parts separately (cf12 406-413).
+// Per located frequency h and loop l: the bucket h hashes to, divided by the
+// filter response at h's offset from the bucket centre (a half-bucket tie
+// rounds down, cf12 371-378); then the lower median (rank (L-1)/2) of the
+// real and imaginary parts separately (cf12 406-413).
+//
+// Blocks are mapped so that every block of one signal lands on the same XCD
+// (workgroups are dispatched round-robin over the 8 XCDs): the signal's
+// buckets then stay in that XCD's L2 while its located set is estimated.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ bool xcd_block(int64_t b, int S, int64_t per_signal,
+                                          int &s, int64_t &c)
+{
+  if (S < kXcds) {  // too few signals to give each XCD its own: spread them
+    s = (int)(b / per_signal);
+    c = b % per_signal;
+    return s < S;
+  }
+  const int x = (int)(b % kXcds);
+  const int64_t q = b / kXcds;
+  s = x + kXcds * (int)(q / per_signal);
+  c = q % per_signal;
+  return s < S;
+}
+
+__host__ __device__ inline int64_t xcd_grid(int S, int64_t per_signal)
+{
+  return S < kXcds ? (int64_t)S * per_signal
+                   : (int64_t)((S + kXcds - 1) / kXcds) * kXcds * per_signal;
+}
+
+__host__ __device__ constexpr int pow2_at_least(int v)
+{
+  return v <= 1 ? 1 : 2 * pow2_at_least((v + 1) / 2);
+}
+
+// Ascending sort of v[0..LMAX) with Batcher's odd-even merge network on the
+// next power of two; comparators that touch a slot >= LMAX compare with an
+// implicit +inf and are dropped at compile time.
 template <int LMAX>
 __device__ __forceinline__ void sort_net(double (&v)[LMAX])
 {
-  // bitonic sorting network, ascending
+  constexpr int N = pow2_at_least(LMAX);
 #pragma unroll
-  for (int k2 = 2; k2 <= LMAX; k2 <<= 1) {
+  for (int p = 1; p < N; p <<= 1) {
 #pragma unroll
-    for (int j = k2 >> 1; j > 0; j >>= 1) {
+    for (int k = p; k >= 1; k >>= 1) {
 #pragma unroll
-      for (int i = 0; i < LMAX; i++) {
-        const int p = i ^ j;
-        if (p > i) {
-          const bool up = (i & k2) == 0;
-          const double lo = fmin(v[i], v[p]), hi = fmax(v[i], v[p]);
-          v[i] = up ? lo : hi;
-          v[p] = up ? hi : lo;
+      for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+          const int lo = i + j, hi = i + j + k;
+          if (hi < LMAX && i < N - j - k && lo / (2 * p) == hi / (2 * p)) {
+            const double x = fmin(v[lo], v[hi]), y = fmax(v[lo], v[hi]);
+            v[lo] = x;
+            v[hi] = y;
+          }
         }
       }
     }
   }
 }
 
-template <int LMAX, bool FP32>
-__global__ void __launch_bounds__(kBlock) k_estimate(V12Launch a)
+template <int LMAX>
+__device__ __forceinline__ double2 lower_median(double (&vr)[LMAX], double (&vi)[LMAX],
+                                                int L)
 {
-  __shared__ unsigned s_ai[kMaxLoops];
-  const int64_t chunks = (a.est_cap + blockDim.x - 1) / blockDim.x;
-  const int s = (int)(blockIdx.x / chunks);
-  const int64_t t = (blockIdx.x % chunks) * blockDim.x + threadIdx.x;
-  const int stride = 2 * a.L + max(a.comb_loops, 1);
-  if (threadIdx.x < a.L)
-    s_ai[threadIdx.x] = (unsigned)a.params[(int64_t)s * stride + a.L + threadIdx.x];
-  __syncthreads();
-  const int64_t cnt = a.count[s];
-  if (t >= cnt) return;
-  const int n = a.n;
-  const unsigned mask = (unsigned)n - 1;
-  unsigned h;
-  if (a.comb_loops > 0) {
-    const int nc = a.num_comb[s];
-    const int64_t j = t / nc;
-    const int i = (int)(t % nc);
-    h = (unsigned)(j * a.W + a.approved[(int64_t)s * a.comb_loops * a.num + i]);
-  } else {
-    h = (unsigned)a.hits[(int64_t)s * a.hits_cap + t];
-  }
-  double vr[LMAX], vi[LMAX];
-#pragma unroll
-  for (int l = 0; l < LMAX; l++) {
-    if (l < a.L) {
-      const bool isloc = l < a.Lloc;
-      const int B = isloc ? a.B_loc : a.B_est;
-      const int m = n / B;
-      const unsigned p = (s_ai[l] * h) & mask;
-      int hashed = (int)(p / (unsigned)m);
-      int dist = (int)(p % (unsigned)m);
-      if (dist > m / 2) {
-        hashed = (hashed + 1) & (B - 1);
-        dist -= m;
-      }
-      const double2 G = (isloc ? a.win_loc : a.win_est)[dist + m / 2];
-      const int64_t idx = (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est) + hashed;
-      double2 U;
-      if (FP32 && !(a.loc64 && isloc)) {
-        const float2 u = a.xs32[idx];
-        U = make_double2((double)u.x, (double)u.y);
-      } else {
-        U = a.xs64[idx];
-      }
-      const double den = G.x * G.x + G.y * G.y;
-      vr[l] = (U.x * G.x + U.y * G.y) / den;
-      vi[l] = a.ref_conj ? (U.x * G.y - U.y * G.x) / den
-                         : (U.y * G.x - U.x * G.y) / den;
-    } else {
-      vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
-      vi[l] = vr[l];
-    }
-  }
   sort_net<LMAX>(vr);
   sort_net<LMAX>(vi);
-  const int rank = (a.L - 1) / 2;
+  const int rank = (L - 1) / 2;
   double mr = vr[0], mi = vi[0];
 #pragma unroll
   for (int l = 1; l < LMAX; l++) {
@@ -423,7 +408,47 @@ __global__ void __launch_bounds__(kBlock) k_estimate(V12Launch a)
       mi = vi[l];
     }
   }
-  const double2 val = make_double2(mr, mi);
+  return make_double2(mr, mi);
+}
+
+__device__ __forceinline__ double2 load_bucket(const V12Launch &a, bool fp32, bool isloc,
+                                               int64_t idx)
+{
+  if (fp32 && !(a.loc64 && isloc)) {
+    const float2 u = a.xs32[idx];
+    return make_double2((double)u.x, (double)u.y);
+  }
+  return a.xs64[idx];
+}
+
+// bucket / G with the reference's operation order (division by |G|^2)
+__device__ __forceinline__ void divide_by_filter(double2 U, double2 G, double den,
+                                                 bool conj, double &vr, double &vi)
+{
+  vr = (U.x * G.x + U.y * G.y) / den;
+  vi = conj ? (U.x * G.y - U.y * G.x) / den : (U.y * G.x - U.x * G.y) / den;
+}
+
+// Bucket and filter value of frequency h in loop l, computed directly.
+__device__ __forceinline__ void hash_direct(const V12Launch &a, int l, unsigned ai,
+                                            unsigned h, int &hashed, double2 &G)
+{
+  const bool isloc = l < a.Lloc;
+  const int B = isloc ? a.B_loc : a.B_est;
+  const int m = a.n / B;
+  const unsigned p = (ai * h) & ((unsigned)a.n - 1);
+  hashed = (int)(p / (unsigned)m);
+  int dist = (int)(p % (unsigned)m);
+  if (dist > m / 2) {
+    hashed = (hashed + 1) & (B - 1);
+    dist -= m;
+  }
+  G = (isloc ? a.win_loc : a.win_est)[dist + m / 2];
+}
+
+__device__ __forceinline__ void write_result(const V12Launch &a, int s, int64_t t,
+                                             unsigned h, double2 val)
+{
   if (a.o_freq && t < a.o_cap) {
     int32_t *of = a.o_freq[s];
     double2 *ov = a.o_val[s];
@@ -434,6 +459,213 @@ __global__ void __launch_bounds__(kBlock) k_estimate(V12Launch a)
     double2 *od = a.o_dense[s];
     if (od) od[h] = val;
   }
+}
+
+// v2: the located set is h = j*W + c for the comb residues c (in the
+// reference's prefill order t = j*num_comb + i).  When W is a multiple of
+// m = n/B, every h with the same residue has the same offset from its
+// bucket centre and its bucket advances by (ai*W/m) mod B per alias row j,
+// so the filter value and base bucket are tabulated per (residue, loop).
+// Block tile: kTI residues x kTJ alias rows, kTI*kTJ/256 hits per thread.
+constexpr int kTI = 64, kTJ = 32;
+
+template <int LMAX, bool FP32, bool ALLFAST>
+__global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles_i,
+                                                          int tiles_j)
+{
+  __shared__ int t_hash[kTI * LMAX];
+  __shared__ double2 t_g[kTI * LMAX];
+  __shared__ double t_den[FP32 ? 1 : kTI * LMAX];
+  __shared__ unsigned s_ai[LMAX];
+  __shared__ unsigned s_step[LMAX];
+  __shared__ unsigned s_mask[LMAX];
+  __shared__ int s_off[LMAX];
+  int s;
+  int64_t c;
+  if (!xcd_block(blockIdx.x, a.S, (int64_t)tiles_i * tiles_j, s, c)) return;
+  const int ti = (int)(c % tiles_i), tj = (int)(c / tiles_i);
+  const int nc = a.num_comb[s];
+  const int rows = a.n / a.W;
+  if (ti * kTI >= nc || tj * kTJ >= rows) return;
+  const int L = a.L;
+  const int stride = 2 * L + max(a.comb_loops, 1);
+  const int *approved = a.approved + (int64_t)s * a.comb_loops * a.num;
+  if (threadIdx.x < L) {
+    const int l = threadIdx.x;
+    const unsigned ai = (unsigned)a.params[(int64_t)s * stride + L + l];
+    const int B = l < a.Lloc ? a.B_loc : a.B_est;
+    const int m = a.n / B;
+    s_ai[l] = ai;
+    s_mask[l] = (unsigned)(B - 1);
+    s_off[l] = (int)loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+    // 0 marks a loop whose W is not a multiple of m (direct hashing)
+    s_step[l] = (a.W % m == 0) ? ((ai * (unsigned)(a.W / m)) & (unsigned)(B - 1)) | 0x80000000u
+                               : 0u;
+  }
+  __syncthreads();
+  if (ALLFAST) {
+    // every loop tabulated: a lean loop body with per-loop scalars in LDS
+    for (int e = threadIdx.x; e < kTI * L; e += blockDim.x) {
+      const int il = e / L, l = e % L;
+      const int i = ti * kTI + il;
+      if (i < nc) {
+        int hashed;
+        double2 G;
+        hash_direct(a, l, s_ai[l], (unsigned)approved[i], hashed, G);
+        const double den = G.x * G.x + G.y * G.y;
+        t_hash[il * LMAX + l] = hashed;
+        if (FP32) {
+          t_g[il * LMAX + l] = make_double2(G.x / den, G.y / den);
+        } else {
+          t_g[il * LMAX + l] = G;
+          t_den[il * LMAX + l] = den;
+        }
+      }
+    }
+    __syncthreads();
+    const double2 *xs64 = a.xs64 + (int64_t)s * a.xs;
+    const float2 *xs32 = FP32 ? a.xs32 + (int64_t)s * a.xs : nullptr;
+    const bool conj = a.ref_conj;
+    for (int e = threadIdx.x; e < kTI * kTJ; e += kBlock) {
+      const int jl = e / kTI;
+      const int il = (e + jl) % kTI;  // skew: see below
+      const int i = ti * kTI + il;
+      const int j = tj * kTJ + jl;
+      if (i >= nc || j >= rows) continue;
+      const unsigned h = (unsigned)j * (unsigned)a.W + (unsigned)approved[i];
+      double vr[LMAX], vi[LMAX];
+#pragma unroll
+      for (int l = 0; l < LMAX; l++) {
+        if (l < L) {
+          const int hashed =
+              (int)(((unsigned)t_hash[il * LMAX + l] + (unsigned)j * (s_step[l] & 0x7fffffffu)) &
+                    s_mask[l]);
+          const double2 g = t_g[il * LMAX + l];
+          double2 U;
+          if (FP32) {
+            const float2 u = xs32[s_off[l] + hashed];
+            U = make_double2((double)u.x, (double)u.y);
+            // g = G / |G|^2: one complex multiply instead of two divisions
+            vr[l] = U.x * g.x + U.y * g.y;
+            const double q = U.x * g.y - U.y * g.x;
+            vi[l] = conj ? q : -q;
+          } else {
+            U = xs64[s_off[l] + hashed];
+            const double den = t_den[il * LMAX + l];
+            vr[l] = (U.x * g.x + U.y * g.y) / den;
+            const double q = (U.x * g.y - U.y * g.x) / den;
+            vi[l] = conj ? q : -q;
+          }
+        } else {
+          vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
+          vi[l] = vr[l];
+        }
+      }
+      write_result(a, s, (int64_t)j * nc + i, h, lower_median<LMAX>(vr, vi, L));
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < kTI * L; e += blockDim.x) {
+    const int il = e / L, l = e % L;
+    const int i = ti * kTI + il;
+    if (i < nc && (s_step[l] & 0x80000000u)) {
+      int hashed;
+      double2 G;
+      hash_direct(a, l, s_ai[l], (unsigned)approved[i], hashed, G);
+      const double den = G.x * G.x + G.y * G.y;
+      t_hash[il * LMAX + l] = hashed;
+      if (FP32) {
+        t_g[il * LMAX + l] = make_double2(G.x / den, G.y / den);
+      } else {
+        t_g[il * LMAX + l] = G;
+        t_den[il * LMAX + l] = den;
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t sbase = (int64_t)s * a.xs;
+  // Hit e of the tile: row jl = e / kTI, residue il skewed by the row so that
+  // a thread's table entries change from hit to hit (keeps them in LDS rather
+  // than hoisted into registers) while a wavefront still writes consecutive
+  // output slots.
+  for (int e = threadIdx.x; e < kTI * kTJ; e += kBlock) {
+    const int jl = e / kTI;
+    const int il = (e + jl) % kTI;
+    const int i = ti * kTI + il;
+    const int j = tj * kTJ + jl;
+    if (i >= nc || j >= rows) continue;
+    const unsigned h = (unsigned)j * (unsigned)a.W + (unsigned)approved[i];
+    double vr[LMAX], vi[LMAX];
+#pragma unroll
+    for (int l = 0; l < LMAX; l++) {
+      if (l < L) {
+        const bool isloc = l < a.Lloc;
+        const int B = isloc ? a.B_loc : a.B_est;
+        const int64_t off = sbase + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+        const unsigned step = s_step[l];
+        if (step & 0x80000000u) {
+          const int hashed = (t_hash[il * LMAX + l] +
+                              (int)(((unsigned)j * (step & 0x7fffffffu)) & (unsigned)(B - 1))) &
+                             (B - 1);
+          const double2 U = load_bucket(a, FP32, isloc, off + hashed);
+          const double2 g = t_g[il * LMAX + l];
+          if (FP32) {
+            // g = G / |G|^2: one complex multiply instead of two divisions
+            vr[l] = U.x * g.x + U.y * g.y;
+            vi[l] = a.ref_conj ? U.x * g.y - U.y * g.x : U.y * g.x - U.x * g.y;
+          } else {
+            divide_by_filter(U, g, t_den[il * LMAX + l], a.ref_conj, vr[l], vi[l]);
+          }
+        } else {
+          int hashed;
+          double2 G;
+          hash_direct(a, l, s_ai[l], h, hashed, G);
+          const double2 U = load_bucket(a, FP32, isloc, off + hashed);
+          divide_by_filter(U, G, G.x * G.x + G.y * G.y, a.ref_conj, vr[l], vi[l]);
+        }
+      } else {
+        vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
+        vi[l] = vr[l];
+      }
+    }
+    write_result(a, s, (int64_t)j * nc + i, h, lower_median<LMAX>(vr, vi, L));
+  }
+}
+
+// v1: one thread per hit of the vote list.
+template <int LMAX, bool FP32>
+__global__ void __launch_bounds__(kBlock) k_estimate_hits(V12Launch a, int64_t chunks)
+{
+  __shared__ unsigned s_ai[LMAX];
+  int s;
+  int64_t c;
+  if (!xcd_block(blockIdx.x, a.S, chunks, s, c)) return;
+  const int L = a.L;
+  const int stride = 2 * L + max(a.comb_loops, 1);
+  if (threadIdx.x < L)
+    s_ai[threadIdx.x] = (unsigned)a.params[(int64_t)s * stride + L + threadIdx.x];
+  __syncthreads();
+  const int64_t t = c * blockDim.x + threadIdx.x;
+  if (t >= a.count[s]) return;
+  const unsigned h = (unsigned)a.hits[(int64_t)s * a.hits_cap + t];
+  const int64_t sbase = (int64_t)s * a.xs;
+  double vr[LMAX], vi[LMAX];
+#pragma unroll
+  for (int l = 0; l < LMAX; l++) {
+    if (l < L) {
+      const bool isloc = l < a.Lloc;
+      int hashed;
+      double2 G;
+      hash_direct(a, l, s_ai[l], h, hashed, G);
+      const double2 U = load_bucket(a, FP32, isloc,
+                                    sbase + loop_offset(l, a.Lloc, a.B_loc, a.B_est) + hashed);
+      divide_by_filter(U, G, G.x * G.x + G.y * G.y, a.ref_conj, vr[l], vi[l]);
+    } else {
+      vr[l] = __longlong_as_double(0x7ff0000000000000ll);
+      vi[l] = vr[l];
+    }
+  }
+  write_result(a, s, t, h, lower_median<LMAX>(vr, vi, L));
 }
 
 __global__ void k_zero(double2 *out, int64_t n)
@@ -512,17 +744,48 @@ int select_vote_v1(const V12Launch &a, hipStream_t st)
   return launched("vote_v1");
 }
 
+template <int LMAX>
+void launch_estimate(const V12Launch &a, hipStream_t st)
+{
+  if (a.comb_loops > 0) {
+    const int tiles_i = (a.comb_loops * a.num + kTI - 1) / kTI;
+    const int tiles_j = (a.n / a.W + kTJ - 1) / kTJ;
+    const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));
+    const bool allfast = a.W % (a.n / a.B_loc) == 0 && a.W % (a.n / a.B_est) == 0;
+    if (a.fp32) {
+      if (allfast)
+        hipLaunchKernelGGL((k_estimate_comb<LMAX, true, true>), grid, dim3(kBlock), 0, st, a,
+                           tiles_i, tiles_j);
+      else
+        hipLaunchKernelGGL((k_estimate_comb<LMAX, true, false>), grid, dim3(kBlock), 0, st,
+                           a, tiles_i, tiles_j);
+    } else {
+      if (allfast)
+        hipLaunchKernelGGL((k_estimate_comb<LMAX, false, true>), grid, dim3(kBlock), 0, st,
+                           a, tiles_i, tiles_j);
+      else
+        hipLaunchKernelGGL((k_estimate_comb<LMAX, false, false>), grid, dim3(kBlock), 0, st,
+                           a, tiles_i, tiles_j);
+    }
+  } else {
+    const int64_t chunks = (a.est_cap + kBlock - 1) / kBlock;
+    const dim3 grid((unsigned)xcd_grid(a.S, chunks));
+    if (a.fp32)
+      hipLaunchKernelGGL((k_estimate_hits<LMAX, true>), grid, dim3(kBlock), 0, st, a, chunks);
+    else
+      hipLaunchKernelGGL((k_estimate_hits<LMAX, false>), grid, dim3(kBlock), 0, st, a, chunks);
+  }
+}
+
 int estimate(const V12Launch &a, hipStream_t st)
 {
-  const int64_t chunks = (a.est_cap + kBlock - 1) / kBlock;
-  const dim3 grid((unsigned)(chunks * a.S));
-  if (a.L <= 16) {
-    if (a.fp32) hipLaunchKernelGGL((k_estimate<16, true>), grid, dim3(kBlock), 0, st, a);
-    else hipLaunchKernelGGL((k_estimate<16, false>), grid, dim3(kBlock), 0, st, a);
-  } else {
-    if (a.fp32) hipLaunchKernelGGL((k_estimate<32, true>), grid, dim3(kBlock), 0, st, a);
-    else hipLaunchKernelGGL((k_estimate<32, false>), grid, dim3(kBlock), 0, st, a);
-  }
+  // the smallest median network that holds the loops
+  if (a.L <= 8) launch_estimate<8>(a, st);
+  else if (a.L <= 12) launch_estimate<12>(a, st);
+  else if (a.L <= 16) launch_estimate<16>(a, st);
+  else if (a.L <= 20) launch_estimate<20>(a, st);
+  else if (a.L <= 24) launch_estimate<24>(a, st);
+  else launch_estimate<32>(a, st);
   return launched("estimate");
 }
 
