@@ -129,10 +129,12 @@ def main():
     # per launch (one batch): gathered samples 16 B each, FP32 taps once,
     # FP32 bucket writes
     fold_bytes = S * (16 * w_sum + 8 * xs_size) + 8 * (prm["w_loc"] + prm["w_est"])
-    fold_ms, fold_launches = stages32["fold_fft"]
+    fold_ms, fold_launches = stages32["fold"]
     fold_avg_ms = fold_ms / max(1, fold_launches)
     achieved = fold_bytes / (fold_avg_ms * 1e-3) / 1e9
-    result["roofline"] = {"kernel": "k_fold_fft32", "bound": "hbm", "achieved": achieved,
+    fused = os.environ.get("SFFT_FOLD_MODE", "1") == "0"
+    kname = "k_fold_fft32" if fused else "k_fold_stream<true>"
+    result["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": None, "alg_bytes_per_launch": fold_bytes,
                           "avg_launch_ms": fold_avg_ms, "launches": fold_launches}

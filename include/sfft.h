@@ -145,11 +145,12 @@ int sfft_derive_params(int n, int k, sfft_version version, int *p10);
  * resets the totals). */
 int sfft_plan_enable_timing(sfft_plan *plan, int enabled);
 
-/* Totals since timing was enabled: for stage s (0 comb, 1 fold+bucket FFT,
- * 2 selection+votes, 3 estimation, 4 v3 loop) ms[s] = summed milliseconds
- * and launches[s] = number of kernel launches timed. */
-int sfft_plan_stage_times(const sfft_plan *plan, double *ms5,
-                          int64_t *launches5);
+/* Totals since timing was enabled: for stage s (0 comb, 1 fold (gather and
+ * bucket sums; includes the bucket FFTs when they are fused into it),
+ * 2 bucket FFTs, 3 selection+votes, 4 estimation, 5 v3) ms[s] = summed
+ * milliseconds and launches[s] = kernel launches timed.  Arrays of 6. */
+int sfft_plan_stage_times(const sfft_plan *plan, double *ms6,
+                          int64_t *launches6);
 
 /* Diagnostics: the bucket values (after the bucket FFTs) of signal `signal`
  * of the plan's last v1/v2 batch, all loops concatenated as the reference's

@@ -376,13 +376,13 @@ int sfft_plan_enable_timing(sfft_plan *plan, int enabled)
   return 0;
 }
 
-int sfft_plan_stage_times(const sfft_plan *plan, double *ms5, int64_t *launches5)
+int sfft_plan_stage_times(const sfft_plan *plan, double *ms6, int64_t *launches6)
 {
   Plan *P = get(plan);
   if (!P) return -1;
   for (int i = 0; i < Plan::kStages; i++) {
-    if (ms5) ms5[i] = P->stage_ms[i];
-    if (launches5) launches5[i] = P->stage_launches[i];
+    if (ms6) ms6[i] = P->stage_ms[i];
+    if (launches6) launches6[i] = P->stage_launches[i];
   }
   return 0;
 }

@@ -194,6 +194,9 @@ Plan *make_plan(int n, int k, sfft_version v)
     delete P;
     return nullptr;
   }
+  // tuning overrides for experiments: fold schedule and its resident blocks
+  if (const char *e = std::getenv("SFFT_FOLD_MODE")) P->fold_mode = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("SFFT_FOLD_BLOCKS")) P->stream_blocks = std::atoi(e);
   P->v12 = derive_v12(n, k, v == SFFT_VERSION_2);
   std::string why = check_v12(P->v12);
   if (!why.empty()) {
@@ -370,6 +373,14 @@ int exec_batch(Plan *P, BatchIO &io)
   a.o_dense = want_dense ? reinterpret_cast<double2 *const *>(dtab + 3 * tab_bytes) : nullptr;
   a.o_cap = io.cap;
   a.est_cap = cap;
+  a.fold_mode = P->fold_mode;
+  a.stream_blocks = P->stream_blocks;
+  if (a.stream_blocks <= 0) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    a.stream_blocks = 2 * cus;  // 8 resident wavefronts per CU
+  }
 
   if (P->timing && !P->ev_made) {
     for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
@@ -378,23 +389,27 @@ int exec_batch(Plan *P, BatchIO &io)
   auto mark = [&](int i) {
     if (P->timing) (void)hipEventRecord(P->ev[i], st);
   };
+  int launches[5] = {0, 0, 0, 0, 0};
   mark(0);
   if (p.comb && k::comb(a, st) != 0) return -1;
+  launches[0] = p.comb ? (cl > 1 ? 2 : 1) : 0;
   mark(1);
   P->last_fp32 = fp32;
   P->last_S = S;
-  if (k::fold_fft(a, st) != 0) return -1;
+  if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
   mark(2);
-  if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
+  if ((launches[2] = k::fold_fft(a, st, 1)) < 0) return -1;
   mark(3);
-  if (k::estimate(a, st) != 0) return -1;
+  if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
+  launches[3] = p.comb ? 0 : 2;
   mark(4);
+  if (k::estimate(a, st) != 0) return -1;
+  launches[4] = 1;
+  mark(5);
   HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
   if (P->timing) {
-    const int launches[4] = {p.comb ? (cl > 1 ? 2 : 1) : 0, loc64 ? 2 : 1,
-                             p.comb ? 0 : 2, 1};
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < 5; i++) {
       float ms = 0;
       if (launches[i] && hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]) == hipSuccess) {
         P->stage_ms[i] += ms;

@@ -96,6 +96,8 @@ struct Plan {
   V12Params v12;
   V3Params v3;
   sfft_precision precision = SFFT_PRECISION_FP64;
+  int fold_mode = 1;       // see k::V12Launch::fold_mode
+  int stream_blocks = 0;   // 0: derived from the CU count
   bool last_fp32 = false;  // precision of the buckets in ws_xsamp
   bool ref_conj = true;
   Rng rng;
@@ -122,10 +124,10 @@ struct Plan {
   size_t pinned_bytes = 0;
 
   // stage timing (hipEvents on the plan's stream)
-  static constexpr int kStages = 5;
+  static constexpr int kStages = 6;
   bool timing = false;
-  double stage_ms[kStages] = {0, 0, 0, 0, 0};
-  int64_t stage_launches[kStages] = {0, 0, 0, 0, 0};
+  double stage_ms[kStages] = {0, 0, 0, 0, 0, 0};
+  int64_t stage_launches[kStages] = {0, 0, 0, 0, 0, 0};
   hipEvent_t ev[kStages + 1] = {};
   bool ev_made = false;
 
@@ -183,9 +185,14 @@ struct V12Launch {
   double2 *const *o_dense;  // S pointers or null
   int64_t o_cap;            // sparse capacity per signal
   int64_t est_cap;          // upper bound of the located count per signal
+  int fold_mode;            // 0: fused fold+FFT per (signal, loop) block;
+                            // 1: signal-major streamed fold, then bucket FFTs
+  int stream_blocks;        // resident blocks of the streamed fold
 };
 int comb(const V12Launch &a, hipStream_t st);
-int fold_fft(const V12Launch &a, hipStream_t st);
+// part 0: the fold (fused with the bucket FFTs when fold_mode == 0);
+// part 1: the bucket FFTs (nothing when fused).  Returns launches or -1.
+int fold_fft(const V12Launch &a, hipStream_t st, int part);
 int select_vote_v1(const V12Launch &a, hipStream_t st);
 int estimate(const V12Launch &a, hipStream_t st);
 int zero_dense(double2 *out, int64_t n, hipStream_t st);

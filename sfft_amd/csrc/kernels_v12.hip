@@ -245,6 +245,148 @@ __global__ void __launch_bounds__(kBlock) k_fold_fft32(V12Launch a, int l0, int 
   for (int j = threadIdx.x; j < B; j += T) out[j] = smem32[j];
 }
 
+// ---------------------------------------------------------------- streamed fold
+// The same bucket sums, scheduled for the Infinity Cache (MALL): work items
+// are (signal, loop, 64 buckets), one per wavefront, numbered signal-major,
+// and a fixed set of resident wavefronts walks them in order.  Only a few
+// signals are then in flight at once, so the second and later touches of a
+// signal's 128-byte lines (20 loops gather ~20% of its samples) find them in
+// the MALL instead of HBM.  Each lane sums its bucket in ascending tap order
+// (identical arithmetic to the fused kernel); the B-point FFT follows in
+// k_bucket_fft.  Buckets are written pre-FFT into xs64/xs32.
+constexpr int kFoldWaves = 4;  // wavefronts per block
+
+struct FoldItems {
+  int l0, nl;          // loop range
+  int c_loc, c_est;    // 64-bucket chunks per location / estimation loop
+  int n_loc;           // location loops in the range
+  int64_t per_signal;  // items per signal
+  int64_t total;
+};
+
+__host__ __device__ inline FoldItems fold_items(const V12Launch &a, int l0, int nl)
+{
+  FoldItems f;
+  f.l0 = l0;
+  f.nl = nl;
+  f.c_loc = (a.B_loc + 63) / 64;
+  f.c_est = (a.B_est + 63) / 64;
+  const int l1 = l0 + nl;
+  f.n_loc = max(0, min(l1, a.Lloc) - l0);
+  f.per_signal = (int64_t)f.n_loc * f.c_loc + (int64_t)(nl - f.n_loc) * f.c_est;
+  f.total = f.per_signal * a.S;
+  return f;
+}
+
+template <bool FP32>
+__global__ void __launch_bounds__(64 * kFoldWaves) k_fold_stream(V12Launch a, FoldItems f)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * kFoldWaves + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * kFoldWaves;
+  const unsigned mask = (unsigned)a.n - 1;
+  const int stride = 2 * a.L + max(a.comb_loops, 1);
+  for (int64_t item = wave; item < f.total; item += nwaves) {
+    const int s = (int)(item / f.per_signal);
+    const int64_t r0 = item % f.per_signal;
+    int l, ch;
+    if (r0 < (int64_t)f.n_loc * f.c_loc) {
+      l = f.l0 + (int)(r0 / f.c_loc);
+      ch = (int)(r0 % f.c_loc);
+    } else {
+      const int64_t r1 = r0 - (int64_t)f.n_loc * f.c_loc;
+      l = f.l0 + f.n_loc + (int)(r1 / f.c_est);
+      ch = (int)(r1 % f.c_est);
+    }
+    const bool isloc = l < a.Lloc;
+    const int B = isloc ? a.B_loc : a.B_est;
+    const int w = isloc ? a.w_loc : a.w_est;
+    const int m = ch * 64 + lane;
+    if (m >= B) continue;
+    const unsigned ai = (unsigned)a.params[(int64_t)s * stride + a.L + l];
+    const double2 *__restrict__ x = a.x[s];
+    const int64_t out = (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est) + m;
+    const int R = (w - m + B - 1) / B;  // taps m, m+B, ... below w
+    if (!FP32) {
+      const double2 *__restrict__ taps = isloc ? a.taps_loc : a.taps_est;
+      double2 acc = make_double2(0.0, 0.0);
+      int r = 0;
+      for (; r + 8 <= R; r += 8) {
+        double2 xv[8], tv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const unsigned i = (unsigned)(m + (r + q) * B);
+          xv[q] = x[(i * ai) & mask];
+          tv[q] = taps[i];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          acc.x = acc.x + (xv[q].x * tv[q].x - xv[q].y * tv[q].y);
+          acc.y = acc.y + (xv[q].x * tv[q].y + xv[q].y * tv[q].x);
+        }
+      }
+      for (; r < R; r++) {
+        const unsigned i = (unsigned)(m + r * B);
+        const double2 xv = x[(i * ai) & mask], tv = taps[i];
+        acc.x = acc.x + (xv.x * tv.x - xv.y * tv.y);
+        acc.y = acc.y + (xv.x * tv.y + xv.y * tv.x);
+      }
+      a.xs64[out] = acc;
+    } else {
+      const float2 *__restrict__ taps = isloc ? a.taps_loc32 : a.taps_est32;
+      float2 sum = make_float2(0.f, 0.f), cmp = make_float2(0.f, 0.f);
+      auto add = [&](double2 xv, float2 tv) {
+        const float2 p = cmul(make_float2((float)xv.x, (float)xv.y), tv);
+        float t = sum.x + p.x, bp = t - sum.x;
+        cmp.x += (sum.x - (t - bp)) + (p.x - bp);
+        sum.x = t;
+        t = sum.y + p.y;
+        bp = t - sum.y;
+        cmp.y += (sum.y - (t - bp)) + (p.y - bp);
+        sum.y = t;
+      };
+      int r = 0;
+      for (; r + 8 <= R; r += 8) {
+        double2 xv[8];
+        float2 tv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const unsigned i = (unsigned)(m + (r + q) * B);
+          xv[q] = x[(i * ai) & mask];
+          tv[q] = taps[i];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) add(xv[q], tv[q]);
+      }
+      for (; r < R; r++) {
+        const unsigned i = (unsigned)(m + r * B);
+        add(x[(i * ai) & mask], taps[i]);
+      }
+      a.xs32[out] = make_float2(sum.x + cmp.x, sum.y + cmp.y);
+    }
+  }
+}
+
+// In-place B-point FFT of every (signal, loop) bucket array of the range.
+template <bool FP32>
+__global__ void __launch_bounds__(kBlock) k_bucket_fft(V12Launch a, int l0, int nl)
+{
+  using T2 = typename std::conditional<FP32, float2, double2>::type;
+  extern __shared__ char smem_raw[];
+  T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+  const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
+  const int B = l < a.Lloc ? a.B_loc : a.B_est;
+  const int logB = ilog2(B);
+  T2 *io;
+  if constexpr (FP32) io = a.xs32; else io = a.xs64;
+  io += (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+  for (int m = threadIdx.x; m < B; m += blockDim.x) buf[bitrev(m, logB)] = io[m];
+  __syncthreads();
+  if constexpr (FP32) lds_fft(buf, logB, a.tw32, a.tw_nmax / B);
+  else lds_fft(buf, logB, a.tw64, a.tw_nmax / B);
+  for (int m = threadIdx.x; m < B; m += blockDim.x) io[m] = buf[m];
+}
+
 // ---------------------------------------------------------------- v1 votes
 // One block per (signal, location loop): top-num buckets by |.|^2.
 // LDS: B doubles.  v1 location loops are always FP64 (xs64): their top-2k
@@ -712,25 +854,66 @@ int comb(const V12Launch &a, hipStream_t st)
   return 0;
 }
 
-int fold_fft(const V12Launch &a, hipStream_t st)
+int fold_fft(const V12Launch &a, hipStream_t st, int part)
 {
   // loops [0, n64) in FP64, [n64, L) in FP32
   const int n64 = !a.fp32 ? a.L : (a.loc64 ? a.Lloc : 0);
   const int B64 = n64 == 0 ? 0 : (n64 > a.Lloc ? max(a.B_loc, a.B_est) : a.B_loc);
   const int B32 = n64 >= a.L ? 0 : (n64 == 0 ? max(a.B_loc, a.B_est) : a.B_est);
+  int launches = 0;
+  if (a.fold_mode == 1) {
+    const dim3 blk(64 * kFoldWaves);
+    if (part == 0) {
+      if (n64 > 0) {
+        const FoldItems f = fold_items(a, 0, n64);
+        const int64_t g =
+            std::min<int64_t>(a.stream_blocks, (f.total + kFoldWaves - 1) / kFoldWaves);
+        hipLaunchKernelGGL(k_fold_stream<false>, dim3((unsigned)g), blk, 0, st, a, f);
+        if (launched("fold_stream64") != 0) return -1;
+        launches++;
+      }
+      if (n64 < a.L) {
+        const FoldItems f = fold_items(a, n64, a.L - n64);
+        const int64_t g =
+            std::min<int64_t>(a.stream_blocks, (f.total + kFoldWaves - 1) / kFoldWaves);
+        hipLaunchKernelGGL(k_fold_stream<true>, dim3((unsigned)g), blk, 0, st, a, f);
+        if (launched("fold_stream32") != 0) return -1;
+        launches++;
+      }
+    } else {
+      if (n64 > 0) {
+        allow_lds(k_bucket_fft<false>, (size_t)B64 * sizeof(double2));
+        hipLaunchKernelGGL(k_bucket_fft<false>, dim3(a.S * n64), dim3(kBlock),
+                           (size_t)B64 * sizeof(double2), st, a, 0, n64);
+        if (launched("bucket_fft64") != 0) return -1;
+        launches++;
+      }
+      if (n64 < a.L) {
+        allow_lds(k_bucket_fft<true>, (size_t)B32 * sizeof(float2));
+        hipLaunchKernelGGL(k_bucket_fft<true>, dim3(a.S * (a.L - n64)), dim3(kBlock),
+                           (size_t)B32 * sizeof(float2), st, a, n64, a.L - n64);
+        if (launched("bucket_fft32") != 0) return -1;
+        launches++;
+      }
+    }
+    return launches;
+  }
+  if (part == 1) return 0;  // fused into the fold
   if (n64 > 0) {
     allow_lds(k_fold_fft64, (size_t)B64 * sizeof(double2));
     hipLaunchKernelGGL(k_fold_fft64, dim3(a.S * n64), dim3(kBlock),
                        (size_t)B64 * sizeof(double2), st, a, 0, n64);
     if (launched("fold_fft64") != 0) return -1;
+    launches++;
   }
   if (n64 < a.L) {
     allow_lds(k_fold_fft32, (size_t)B32 * sizeof(float2));
     hipLaunchKernelGGL(k_fold_fft32, dim3(a.S * (a.L - n64)), dim3(kBlock),
                        (size_t)B32 * sizeof(float2), st, a, n64, a.L - n64);
     if (launched("fold_fft32") != 0) return -1;
+    launches++;
   }
-  return 0;
+  return launches;
 }
 
 int select_vote_v1(const V12Launch &a, hipStream_t st)

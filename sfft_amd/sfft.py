@@ -213,15 +213,15 @@ class sfft:
             raise SfftError(_lib.last_error())
         return np.array([outs[i].count for i in range(num)], dtype=np.int64)
 
-    STAGES = ["comb", "fold_fft", "select_vote", "estimate", "v3_loop"]
+    STAGES = ["comb", "fold", "bucket_fft", "select_vote", "estimate", "v3"]
 
     def enable_timing(self, enabled=True):
         self._L.sfft_plan_enable_timing(self.sfft_plan, int(bool(enabled)))
 
     def stage_times(self):
         """{stage: (total_ms, launches)} from the library's HIP events."""
-        ms = (C.c_double * 5)()
-        ln = (C.c_int64 * 5)()
+        ms = (C.c_double * 6)()
+        ln = (C.c_int64 * 6)()
         self._L.sfft_plan_stage_times(self.sfft_plan, ms, ln)
         return {s: (ms[i], ln[i]) for i, s in enumerate(self.STAGES)}
 
