@@ -103,6 +103,7 @@ def lib():
         L.or_v3_plan_make.argtypes = [C.c_int, C.c_int]
         L.or_v3_plan_make.restype = C.POINTER(V3Plan)
         L.or_v3_plan_free.argtypes = [C.POINTER(V3Plan)]
+        L.or_v3_plan_set_filters.argtypes = [C.POINTER(V3Plan), P, P, P, P]
         L.or_v3_draw.argtypes = [C.c_int, P]
         L.or_v3_exec.argtypes = [C.POINTER(V3Plan), P, P, P, P, P, P, C.c_long]
         L.or_v3_exec.restype = C.c_long
@@ -268,6 +269,13 @@ class V3:
         d = np.zeros(6, dtype=np.int32)
         lib().or_v3_draw(self.n, _p(d))
         return d
+
+    def set_filters(self, time1, freq1, time2, freq2):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128)
+                for a in (time1, freq1, time2, freq2)]
+        assert len(arrs[0]) == self.p.w_g1 and len(arrs[2]) == self.p.w_g2
+        assert len(arrs[1]) == self.n and len(arrs[3]) == self.n
+        lib().or_v3_plan_set_filters(self._h, *[_p(a) for a in arrs])
 
     def exec(self, x, draws):
         x = np.ascontiguousarray(x, dtype=np.complex128)

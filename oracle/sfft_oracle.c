@@ -792,6 +792,17 @@ or_v3_plan *or_v3_plan_make(int n, int k)
   return P;
 }
 
+void or_v3_plan_set_filters(or_v3_plan *P, const or_cplx *time1,
+                            const or_cplx *freq1, const or_cplx *time2,
+                            const or_cplx *freq2)
+{
+  size_t N = (size_t)P->p.n;
+  memcpy(P->time1, time1, sizeof(or_cplx) * (size_t)P->p.w_g1);
+  memcpy(P->freq1, freq1, sizeof(or_cplx) * N);
+  memcpy(P->time2, time2, sizeof(or_cplx) * (size_t)P->p.w_g2);
+  memcpy(P->freq2, freq2, sizeof(or_cplx) * N);
+}
+
 void or_v3_plan_free(or_v3_plan *P)
 {
   if (!P) return;
@@ -1145,8 +1156,11 @@ long or_v3_exec(const or_v3_plan *P, const or_cplx *x, const int *d,
     int w2 = p->w_g2;
     or_cplx *perm = (or_cplx *)malloc(sizeof(or_cplx) * (size_t)(w2 + 2));
     double th0 = 2 * M_PI * gG * b / n, th1 = 2 * M_PI * ai * b / n;
-    or_cplx sh = CMPLX(cos(th0), sin(th0));
-    or_cplx st = CMPLX(cos(th1), sin(th1));
+    double s0, c0, s1, c1;
+    sincos(th0, &s0, &c0); /* explicit: see fft_pow2_inplace */
+    sincos(th1, &s1, &c1);
+    or_cplx sh = CMPLX(c0, s0);
+    or_cplx st = CMPLX(c1, s1);
     int idx = gG % n;
     for (int i = 0; i < w2 + 1; i++) {
       double xr = creal(x[idx]), xi = cimag(x[idx]);

@@ -126,6 +126,10 @@ typedef struct {
 
 or_v3_plan *or_v3_plan_make(int n, int k);
 void or_v3_plan_free(or_v3_plan *P);
+/* Replace the plan's two filters (w_g1/w_g2 taps, n-point responses). */
+void or_v3_plan_set_filters(or_v3_plan *P, const or_cplx *time1,
+                            const or_cplx *freq1, const or_cplx *time2,
+                            const or_cplx *freq2);
 
 /* RNG draws of one v3 exec (cf3 800-810): a, ai, b, shift, offsets. */
 void or_v3_draw(int n, int *draws6);

@@ -50,10 +50,6 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
         sfft_output_c *sparse)
 {
   if (num <= 0) return 0;
-  if (P->version == SFFT_VERSION_3) {
-    set_error("sFFT v3 is not available in this build yet");
-    return -1;
-  }
   const int64_t n = P->n;
   const int64_t cap = P->output_capacity();
   bool host_in = false;

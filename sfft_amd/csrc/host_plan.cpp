@@ -143,7 +143,7 @@ static std::vector<double2> est_window(const std::vector<cd> &freq, int n, int B
 // ---------------------------------------------------------------- plan
 int64_t Plan::output_capacity() const
 {
-  if (version == SFFT_VERSION_3) return 8LL * v3.k + 4LL * v3.W_Man + 64;
+  if (version == SFFT_VERSION_3) return v3_Mcap;
   const V12Params &p = v12;
   if (p.comb) return (int64_t)p.comb_loops * p.B_thresh * (p.n / p.W_comb);
   // v1: every hit is selected in >= thr of the location loops.
@@ -161,11 +161,76 @@ int64_t Plan::plan_bytes() const
 
 int64_t Plan::bytes_per_signal() const
 {
-  if (version == SFFT_VERSION_3) return 64LL * 1024 + output_capacity() * 24;
+  if (version == SFFT_VERSION_3)
+    return 32LL * v3.W_Man + (int64_t)v3_Mcap * 28 + (int64_t)v3_Rcap * 236 +
+           (int64_t)v3_Fcap * 40 + 16LL * (v3.w_g2 + 2) + 512;
   const V12Params &p = v12;
   return p.x_samp_size() * 16 + (int64_t)p.comb_loops * p.B_thresh * 4 +
          (int64_t)p.loc_loops * p.B_thresh * 4 + output_capacity() * (4 + 16 + 4) +
          (2 * p.loops() + p.comb_loops + 8) * 4;
+}
+
+// Filter response window at signed offsets -half..half (freq[d mod n]);
+// v3 reads offsets within 3n/2B of a bucket centre (cf3 371-375, 611-613).
+static std::vector<double2> centred_window(const std::vector<cd> &freq, int n, int half)
+{
+  std::vector<double2> w(2 * (size_t)half + 1);
+  for (int d = -half; d <= half; d++) {
+    const cd v = freq[(size_t)(((int64_t)d % n + n) % n)];
+    w[d + half] = make_double2(v.real(), v.imag());
+  }
+  return w;
+}
+
+static std::vector<cd> twiddles(int nmax)
+{
+  std::vector<cd> tw(nmax / 2);
+  for (int j = 0; j < nmax / 2; j++) {
+    // sincos() explicitly: whether a compiler fuses sin+cos into it varies,
+    // and the two differ in the last bit for rare arguments.
+    const double ang = 2.0 * M_PI * (double)j / (double)nmax;
+    double s, c;
+    ::sincos(ang, &s, &c);
+    tw[j] = cd(c, -s);
+  }
+  return tw;
+}
+
+static int pow2_ceil(int64_t v)
+{
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// v3 plan tables (src/sfft.cc:513-560): two filters with b = n/B.
+static int make_v3_tables(Plan *P)
+{
+  const V3Params &q = P->v3;
+  const int n = q.n;
+  HostFilter f1 = design_filter(0.5 / q.BB1, 1e-8, n, q.b_g1);
+  HostFilter f2 = design_filter(0.5 / q.BB2, 1e-8, n, q.b_g2);
+  P->v3_half1 = 3 * (n / q.B_g1) / 2;
+  P->v3_half2 = 3 * (n / q.B_g2) / 2;
+  P->tw_nmax = std::max(q.W_Man, std::max(q.B_g1, q.B_g2));
+  // map of located frequencies: room for several passes' spurious finds
+  P->v3_Mcap = pow2_ceil(8LL * q.k + q.W_Man + 2LL * q.B_g1 + 2LL * q.B_g2 + 64);
+  P->v3_Hcap = 2 * P->v3_Mcap;
+  P->v3_Fcap = std::max(q.W_Man, std::max(q.B_g1, q.B_g2));
+  P->v3_Rcap = std::max(P->v3_Mcap, P->v3_Fcap);
+  P->v3_lds = std::max<size_t>((size_t)q.W_Man * 16,
+                               (size_t)(2 * q.B_g1 + 2 * q.B_g2) * 16);
+  P->v3_lds = std::max<size_t>(P->v3_lds, std::min<size_t>((size_t)P->v3_Mcap * 8, 131072));
+  if (P->v3_lds > 160 * 1024) {
+    set_error("v3 Mansour filter above 8192 buckets is not supported yet");
+    return -1;
+  }
+  const bool ok = upload(P->v3_time1, to_d2(f1.time, f1.time.size())) &&
+                  upload(P->v3_time2, to_d2(f2.time, f2.time.size())) &&
+                  upload(P->v3_freq1, centred_window(f1.freq, n, P->v3_half1)) &&
+                  upload(P->v3_freq2, centred_window(f2.freq, n, P->v3_half2)) &&
+                  upload(P->tw64, to_d2(twiddles(P->tw_nmax), P->tw_nmax / 2));
+  return ok ? 0 : -1;
 }
 
 Plan *make_plan(int n, int k, sfft_version v)
@@ -190,9 +255,13 @@ Plan *make_plan(int n, int k, sfft_version v)
       delete P;
       return nullptr;
     }
-    set_error("sFFT v3 is not available in this build yet");
-    delete P;
-    return nullptr;
+    if (make_v3_tables(P) != 0 || hipStreamCreate(&P->stream) != hipSuccess) {
+      if (P->stream == nullptr && sfa::last_error()[0] == 0) set_error("hipStreamCreate failed");
+      free_plan(P);
+      return nullptr;
+    }
+    P->own_stream = true;
+    return P;
   }
   // tuning overrides for experiments: fold schedule and its resident blocks
   if (const char *e = std::getenv("SFFT_FOLD_MODE")) P->fold_mode = std::atoi(e) ? 1 : 0;
@@ -209,15 +278,7 @@ Plan *make_plan(int n, int k, sfft_version v)
   HostFilter fe = design_filter(0.5 / p.BB_est, p.tol_est, p.n, p.b_est);
   int nmax = std::max(p.B_loc, p.B_est);
   if (p.comb) nmax = std::max(nmax, p.W_comb);
-  std::vector<cd> tw(nmax / 2);
-  for (int j = 0; j < nmax / 2; j++) {
-    // sincos() explicitly: whether a compiler fuses sin+cos into it varies,
-    // and the two differ in the last bit for rare arguments.
-    const double ang = 2.0 * M_PI * (double)j / (double)nmax;
-    double s, c;
-    ::sincos(ang, &s, &c);
-    tw[j] = cd(c, -s);
-  }
+  const std::vector<cd> tw = twiddles(nmax);
   P->tw_nmax = nmax;
   bool ok = upload(P->taps_loc, to_d2(fl.time, fl.time.size())) &&
             upload(P->taps_est, to_d2(fe.time, fe.time.size())) &&
@@ -267,8 +328,149 @@ static int ensure_pinned(Plan *P, size_t bytes)
   return 0;
 }
 
+// v3: draws (cf3 800-810) in signal order, one workgroup per signal.
+static int exec_batch_v3(Plan *P, BatchIO &io)
+{
+  const int S = io.S;
+  const V3Params &q = P->v3;
+  const int n = q.n;
+  const size_t param_bytes = ((size_t)S * 6 * sizeof(int) + 255) & ~size_t(255);
+  const size_t tab_bytes = ((size_t)S * sizeof(void *) + 255) & ~size_t(255);
+  const size_t cnt_bytes = ((size_t)S * sizeof(int64_t) + 255) & ~size_t(255);
+  if (ensure_pinned(P, param_bytes + 4 * tab_bytes + cnt_bytes + (size_t)S * 4) != 0) return -1;
+  char *pin = reinterpret_cast<char *>(P->pinned);
+  int *hp = reinterpret_cast<int *>(pin);
+  const void **t_in = reinterpret_cast<const void **>(pin + param_bytes);
+  const void **t_freq = reinterpret_cast<const void **>(pin + param_bytes + tab_bytes);
+  const void **t_val = reinterpret_cast<const void **>(pin + param_bytes + 2 * tab_bytes);
+  const void **t_dense = reinterpret_cast<const void **>(pin + param_bytes + 3 * tab_bytes);
+  int64_t *h_counts = reinterpret_cast<int64_t *>(pin + param_bytes + 4 * tab_bytes);
+  int *h_status = reinterpret_cast<int *>(pin + param_bytes + 4 * tab_bytes + cnt_bytes);
+  for (int s = 0; s < S; s++) {
+    int *d = hp + (size_t)s * 6;
+    int a = 0;
+    while (gcd_int(a, n) != 1) a = (int)(P->rng.next_random() % n);
+    const int ai = mod_inverse_int(a, n);
+    const int b = (int)(P->rng.next_random() % n);
+    d[0] = a;
+    d[1] = ai;
+    d[2] = b;
+    d[3] = (int)(((int64_t)ai * b) % n);  // ((ai*b) % n + n) % n with int wrap
+    d[4] = (int)(unsigned)std::floor(P->rng.next_drand48() * n);
+    d[5] = (int)(unsigned)std::floor(P->rng.next_drand48() * n);
+    t_in[s] = io.in[s];
+    t_freq[s] = io.freq ? io.freq[s] : nullptr;
+    t_val[s] = io.val ? io.val[s] : nullptr;
+    t_dense[s] = io.dense ? io.dense[s] : nullptr;
+  }
+  // workspace: each array is S x (its per-signal count), stride = that count
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t Sz = (size_t)S;
+  const size_t b_ms = al(Sz * 2 * q.W_Man * 16), b_keys = al(Sz * P->v3_Mcap * 4),
+               b_vals = al(Sz * P->v3_Mcap * 16), b_hidx = al(Sz * P->v3_Hcap * 4),
+               b_fi = al(Sz * P->v3_Fcap * 4), b_fv = al(Sz * P->v3_Fcap * 16),
+               b_rem = al(Sz * P->v3_Rcap * 14 * 16), b_remi = al(Sz * P->v3_Rcap * 3 * 4);
+  const int Pcap = q.w_g2 + 2;
+  const size_t b_ph = al(Sz * Pcap * 16);
+  const size_t total =
+      b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi + b_ph;
+  if (!P->ws_params.ensure(param_bytes) || !P->ws_ptrs.ensure(4 * tab_bytes) ||
+      !P->ws_v3.ensure(total) || !P->ws_count.ensure(cnt_bytes) ||
+      !P->ws_v3_status.ensure((size_t)S * 4))
+    return -1;
+  hipStream_t st = P->stream;
+  HIP_OK(hipMemcpyAsync(P->ws_params.p, hp, (size_t)S * 6 * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(P->ws_ptrs.p, pin + param_bytes, 4 * tab_bytes,
+                        hipMemcpyHostToDevice, st));
+  char *dtab = reinterpret_cast<char *>(P->ws_ptrs.p);
+  char *w = reinterpret_cast<char *>(P->ws_v3.p);
+  k::V3Launch a{};
+  a.n = n;
+  a.S = S;
+  a.k = q.k;
+  a.W = q.W_Man;
+  a.B1 = q.B_g1;
+  a.B2 = q.B_g2;
+  a.w1 = q.w_g1;
+  a.w2 = q.w_g2;
+  a.x = reinterpret_cast<const double2 *const *>(dtab);
+  a.params = P->ws_params.as<const int>();
+  a.taps1 = P->v3_time1.as<const double2>();
+  a.taps2 = P->v3_time2.as<const double2>();
+  a.win1 = P->v3_freq1.as<const double2>();
+  a.win2 = P->v3_freq2.as<const double2>();
+  a.win1_half = P->v3_half1;
+  a.win2_half = P->v3_half2;
+  a.tw64 = P->tw64.as<const double2>();
+  a.tw_nmax = P->tw_nmax;
+  a.ms = reinterpret_cast<double2 *>(w);
+  w += b_ms;
+  a.map_keys = reinterpret_cast<int *>(w);
+  w += b_keys;
+  a.map_vals = reinterpret_cast<double2 *>(w);
+  w += b_vals;
+  a.map_hidx = reinterpret_cast<int *>(w);
+  w += b_hidx;
+  a.cf = reinterpret_cast<int *>(w);
+  w += b_fi;
+  a.ef = reinterpret_cast<int *>(w);
+  w += b_fi;
+  a.cv = reinterpret_cast<double2 *>(w);
+  w += b_fv;
+  a.ev = reinterpret_cast<double2 *>(w);
+  w += b_fv;
+  a.rem = reinterpret_cast<double2 *>(w);
+  w += b_rem;
+  a.remi = reinterpret_cast<int *>(w);
+  w += b_remi;
+  a.phase = reinterpret_cast<double2 *>(w);
+  a.Pcap = Pcap;
+  a.Mcap = P->v3_Mcap;
+  a.Hcap = P->v3_Hcap;
+  a.Fcap = P->v3_Fcap;
+  a.Rcap = P->v3_Rcap;
+  a.max_passes = 3 * 1024;
+  a.count = P->ws_count.as<int64_t>();
+  a.status = P->ws_v3_status.as<int>();
+  const bool want_sparse = io.freq != nullptr || io.val != nullptr;
+  a.o_freq = want_sparse ? reinterpret_cast<int32_t *const *>(dtab + tab_bytes) : nullptr;
+  a.o_val = want_sparse ? reinterpret_cast<double2 *const *>(dtab + 2 * tab_bytes) : nullptr;
+  a.o_dense = io.dense ? reinterpret_cast<double2 *const *>(dtab + 3 * tab_bytes) : nullptr;
+  a.o_cap = io.cap;
+  a.lds_bytes = P->v3_lds;
+  P->last_S = S;
+  if (P->timing && !P->ev_made) {
+    for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
+    P->ev_made = true;
+  }
+  if (P->timing) (void)hipEventRecord(P->ev[0], st);
+  if (k::v3(a, st) != 0) return -1;
+  if (P->timing) (void)hipEventRecord(P->ev[1], st);
+  HIP_OK(hipMemcpyAsync(h_counts, a.count, (size_t)S * 8, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(h_status, a.status, (size_t)S * 4, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (P->timing) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, P->ev[0], P->ev[1]) == hipSuccess) {
+      P->stage_ms[5] += ms;
+      P->stage_launches[5] += 1;
+    }
+  }
+  P->last_status.assign(h_status, h_status + S);
+  if (io.counts)
+    for (int s = 0; s < S; s++) io.counts[s] = h_counts[s];
+  for (int s = 0; s < S; s++)
+    if (h_status[s] & 2) {
+      set_error("v3 result map overflow (more located frequencies than the plan holds)");
+      return -1;
+    }
+  return 0;
+}
+
 int exec_batch(Plan *P, BatchIO &io)
 {
+  if (P->version == SFFT_VERSION_3) return exec_batch_v3(P, io);
   const int S = io.S;
   const V12Params &p = P->v12;
   const int L = p.loops(), cl = p.comb ? p.comb_loops : 0;

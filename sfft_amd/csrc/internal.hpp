@@ -113,7 +113,12 @@ struct Plan {
 
   // v3 device tables
   DevBuf v3_time1, v3_time2;  // double2[w_g*]
-  DevBuf v3_freq1, v3_freq2;  // double2[n] (full responses, read sparsely)
+  DevBuf v3_freq1, v3_freq2;  // double2[3n/B + 1]: responses at offsets -3n/2B..3n/2B
+  int v3_half1 = 0, v3_half2 = 0;
+  int v3_Mcap = 0, v3_Hcap = 0, v3_Fcap = 0, v3_Rcap = 0;
+  size_t v3_lds = 0;
+  DevBuf ws_v3_status;
+  std::vector<int> last_status;  // per signal of the last v3 batch
 
   // per-exec workspace (grow-only)
   DevBuf ws_params, ws_xsamp, ws_xsamp32, ws_approved, ws_numcomb, ws_J,
@@ -189,6 +194,41 @@ struct V12Launch {
                             // 1: signal-major streamed fold, then bucket FFTs
   int stream_blocks;        // resident blocks of the streamed fold
 };
+struct V3Launch {
+  int n, S, k;
+  int W, B1, B2, w1, w2;
+  const double2 *const *x;   // S signal pointers
+  const int *params;         // S x 6: a, ai, b, shift, init_offset, init_G_offset
+  const double2 *taps1, *taps2;
+  const double2 *win1, *win2;  // filter response at offsets -half..half
+  int win1_half, win2_half;
+  const double2 *tw64;
+  int tw_nmax;
+  // per-signal workspace, strided by signal
+  double2 *ms;               // 2W (interleaved loops)
+  int *map_keys;             // Mcap (insertion order)
+  double2 *map_vals;         // Mcap
+  int *map_hidx;             // Hcap hash slots -> entry
+  int Mcap, Hcap;
+  int *cf, *ef;              // Fcap candidates / found
+  double2 *cv, *ev;
+  int Fcap;
+  double2 *rem;              // Rcap x 14 subtraction values
+  int *remi;                 // Rcap x 3 bucket indices
+  int Rcap;
+  double2 *phase;            // Pcap phases of the permuted samples
+  int Pcap;
+  int max_passes;
+  int64_t *count;            // S located counts
+  int *status;               // S: 1 early return, 2 map full, 4 pass cap
+  int32_t *const *o_freq;
+  double2 *const *o_val;
+  double2 *const *o_dense;
+  int64_t o_cap;
+  size_t lds_bytes;
+};
+int v3(const V3Launch &a, hipStream_t st);
+
 int comb(const V12Launch &a, hipStream_t st);
 // part 0: the fold (fused with the bucket FFTs when fold_mode == 0);
 // part 1: the bucket FFTs (nothing when fused).  Returns launches or -1.

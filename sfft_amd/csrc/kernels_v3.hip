@@ -1,0 +1,626 @@
+// sFFT v3 (exactly sparse) for gfx950: one workgroup per signal runs the
+// whole of src/computefourier-3.0.cc:784-1088 -- Mansour filter, two
+// Dolph-Chebyshev "Gauss" filters (the second permuted), and the round-robin
+// estimate/subtract loop.
+//
+// Parallel within a signal: the filters (gathers, folds, FFTs in LDS), the
+// per-bucket estimation (atan2 / sincos), the result-map updates and the
+// subtraction values.  The subtractions themselves are applied in the
+// reference's order (found list order, one wavefront per bucket array), so
+// every bucket sees the same sequence of roundings as in the oracle.  The
+// result map is iterated in insertion order (the reference's
+// tr1::unordered_map order is unreproducible; see the oracle).
+#include <algorithm>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace sfa {
+namespace k {
+
+using namespace sfa::dev;
+
+namespace {
+
+constexpr int kV3Block = 512;
+
+__device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
+
+// int products in the reference wrap modulo 2^32 (cf3 381-382, 475, 743)
+__device__ __forceinline__ int wmul(int x, int y) { return (int)((unsigned)x * (unsigned)y); }
+
+__device__ __forceinline__ int timesmod(int x, int a, int n)
+{
+  return (int)(((long long)x * a) % n);
+}
+
+// Filter response at signed offset d from a window stored as
+// win[d + half], d in [-half, half].
+__device__ __forceinline__ double2 win_at(const double2 *win, int half, unsigned dist, int n)
+{
+  int d = (int)dist;
+  if (d > n / 2) d -= n;
+  return win[d + half];
+}
+
+struct Ctx {
+  const V3Launch *a;
+  int s, n, W, B1, B2;
+  int aa, ai, b, shift, g0, gG;
+  double2 *GS0, *GS1, *GP0, *GP1;  // LDS
+  double2 *MS;                      // global, interleaved [2i + loop]
+  int *keys;
+  double2 *vals;
+  int *hidx;
+  int *cf;
+  double2 *cv;
+  int *ef;
+  double2 *ev;
+  double2 *rem;
+  int *remi;
+};
+
+// ---------------------------------------------------------------- map
+__device__ __forceinline__ unsigned hslot(int key, int hcap)
+{
+  return ((unsigned)key * 2654435761u) & (unsigned)(hcap - 1);
+}
+
+__device__ int map_find(const Ctx &c, int key)
+{
+  const int H = c.a->Hcap;
+  unsigned h = hslot(key, H);
+  for (int probe = 0; probe < H; probe++) {
+    const int e = c.hidx[h];
+    if (e < 0) return -1;
+    if (c.keys[e] == key) return e;
+    h = (h + 1) & (unsigned)(H - 1);
+  }
+  return -1;
+}
+
+__device__ void map_insert_slot(const Ctx &c, int key, int e)
+{
+  const int H = c.a->Hcap;
+  unsigned h = hslot(key, H);
+  for (int probe = 0; probe < H; probe++) {
+    if (atomicCAS(&c.hidx[h], -1, e) == -1) return;
+    h = (h + 1) & (unsigned)(H - 1);
+  }
+}
+
+// ---------------------------------------------------------------- FFT
+// Two B-point forward FFTs of a (x0, x1) pair held in LDS in natural order.
+__device__ void fft_pair(double2 *x0, double2 *x1, int B, const V3Launch &a)
+{
+  const int logB = ilog2v(B);
+  // bit-reverse in place
+  for (int i = threadIdx.x; i < B; i += blockDim.x) {
+    const int r = (int)bitrev((unsigned)i, logB);
+    if (r > i) {
+      double2 t = x0[i];
+      x0[i] = x0[r];
+      x0[r] = t;
+      t = x1[i];
+      x1[i] = x1[r];
+      x1[r] = t;
+    }
+  }
+  __syncthreads();
+  lds_fft(x0, logB, a.tw64, a.tw_nmax / B);
+  lds_fft(x1, logB, a.tw64, a.tw_nmax / B);
+}
+
+// ---------------------------------------------------------------- removals
+// update_gaussian_loops2 (cf3 354-463): what to subtract from buckets
+// h, h+1, h-1 of both loops for (key, value); r[0..5] = loop0 x3, loop1 x3.
+__device__ void gauss_removal(const Ctx &c, int key, double2 value, int B,
+                              const double2 *win, int half, int g, int step,
+                              int &h, double2 *r)
+{
+  const int n = c.n;
+  const unsigned n1 = (unsigned)n - 1;
+  const double PI2_DIV_N = 2 * M_PI / n;
+  const unsigned nb = (unsigned)n / (unsigned)B;
+  h = (int)(key * 1. / nb + 0.5) % B;
+  const double a1 = value.x, b1 = value.y;
+  const unsigned dist1 = ((unsigned)h * nb - (unsigned)key + (unsigned)n) & n1;
+  const unsigned dist2 = (dist1 + (unsigned)n + nb) & n1;
+  const unsigned dist3 = (dist1 + (unsigned)n + (unsigned)n - nb) & n1;
+  const int ko = wmul(key, g) % n;
+  const int ko2 = wmul(key, (int)((unsigned)g + (unsigned)step)) % n;
+  double b2, a2, b22, a22;
+  sincos(PI2_DIV_N * ko, &b2, &a2);
+  sincos(PI2_DIV_N * ko2, &b22, &a22);
+  const double v1r = a1 * a2 - b1 * b2, v1i = a1 * b2 + b1 * a2;
+  const double v2r = a1 * a22 - b1 * b22, v2i = a1 * b22 + b1 * a22;
+  const unsigned dd[3] = {dist1, dist2, dist3};
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const double2 F = win_at(win, half, dd[q], n);
+    r[q] = make_double2(v1r * F.x - v1i * F.y, v1r * F.y + v1i * F.x);
+    r[3 + q] = make_double2(v2r * F.x - v2i * F.y, v2r * F.y + v2i * F.x);
+  }
+}
+
+__device__ void gauss_apply(double2 *S0, double2 *S1, int B, int h, const double2 *r)
+{
+  const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
+  for (int q = 0; q < 3; q++) S0[bk[q]] = csub(S0[bk[q]], r[q]);
+  for (int q = 0; q < 3; q++) S1[bk[q]] = csub(S1[bk[q]], r[3 + q]);
+}
+
+// update_mansour_loops2 (cf3 300-351)
+__device__ void mansour_removal(const Ctx &c, int key, double2 value, int &h, double2 *r)
+{
+  const double PI2_OVER_N = 2 * M_PI / c.n;
+  const int W = c.W;
+  h = key & (W - 1);
+  const double vr = value.x, vi = value.y;
+  double s0, c0, s1, c1;
+  sincos(PI2_OVER_N * key * c.g0, &s0, &c0);
+  sincos(PI2_OVER_N * key * (c.g0 + 1), &s1, &c1);
+  r[0] = make_double2(W * (vr * c0 - vi * s0), W * (vr * s0 + vi * c0));
+  r[1] = make_double2(W * (vr * c1 - vi * s1), W * (vr * s1 + vi * c1));
+}
+
+// Subtraction values of cnt (key, value) pairs, then their application in
+// pair order.  which: bit 0 Gauss-1, bit 1 Gauss-2, bit 2 Mansour.
+// Slots per pair in rem: [0..5] G1, [6..11] G2, [12..13] Mansour.
+__device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int cnt,
+                         int which)
+{
+  const V3Launch &a = *c.a;
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const int key = keys[i];
+    const double2 v = vals[i];
+    double2 *r = c.rem + (int64_t)i * 14;
+    int *ri = c.remi + (int64_t)i * 3;
+    if (which & 1) gauss_removal(c, key, v, c.B1, a.win1, a.win1_half, c.gG, 1, ri[0], r);
+    if (which & 2)
+      gauss_removal(c, (timesmod(key, c.ai, c.n) + c.shift) % c.n, v, c.B2, a.win2,
+                    a.win2_half, wmul(c.aa, c.gG), 1, ri[1], r + 6);
+    if (which & 4) mansour_removal(c, key, v, ri[2], r + 12);
+  }
+  __syncthreads();
+  // one lane per bucket array, in pair order
+  if (threadIdx.x == 0 && (which & 1)) {
+    for (int i = 0; i < cnt; i++)
+      gauss_apply(c.GS0, c.GS1, c.B1, c.remi[i * 3], c.rem + (int64_t)i * 14);
+  } else if (threadIdx.x == 64 && (which & 2)) {
+    for (int i = 0; i < cnt; i++)
+      gauss_apply(c.GP0, c.GP1, c.B2, c.remi[i * 3 + 1], c.rem + (int64_t)i * 14 + 6);
+  } else if (threadIdx.x == 128 && (which & 4)) {
+    for (int i = 0; i < cnt; i++) {
+      const int h = c.remi[i * 3 + 2];
+      const double2 *r = c.rem + (int64_t)i * 14 + 12;
+      c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
+      c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- estimation
+// Candidates per bucket into cf/cv (cf = -1: none), then compaction in
+// bucket order into ef/ev.  Returns the found count (uniform).
+__device__ int compact(const Ctx &c, int nb, int *sh)
+{
+  const int T = blockDim.x;
+  const int C = (nb + T - 1) / T;
+  const int lo = threadIdx.x * C, hi = min(nb, lo + C);
+  int cnt = 0;
+  for (int i = lo; i < hi; i++) cnt += c.cf[i] >= 0;
+  int total;
+  int pos = block_exclusive_scan(cnt, sh, total);
+  for (int i = lo; i < hi; i++) {
+    if (c.cf[i] >= 0) {
+      c.ef[pos] = c.cf[i];
+      c.ev[pos] = c.cv[i];
+      pos++;
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  return total;
+}
+
+// estimate_freq_gauss_loops2 (cf3 484-640) on (S0, S1), B buckets.
+__device__ int est_gauss(const Ctx &c, const double2 *S0, const double2 *S1, int B, int a_,
+                         int b_, const double2 *win, int half, int *sh)
+{
+  const int n = c.n;
+  const unsigned n1 = (unsigned)n - 1, B1m = (unsigned)B - 1;
+  const double PI2 = 2 * M_PI;
+  const double N_OVER_PI2 = n / PI2;
+  const double PI2_A_OFFSET_OVER_N = PI2 * a_ * c.gG / n;
+  const double BUCKETS_OVER_N = (double)B / n;
+  const int N_OVER_BUCKETS = n / B;
+  for (int bk = threadIdx.x; bk < B; bk += blockDim.x) {
+    int out = -1;
+    double2 val = make_double2(0, 0);
+    const double a0 = S0[bk].x, b0 = S0[bk].y, a1 = S1[bk].x, b1 = S1[bk].y;
+    const double zb = (a0 * a0 + b0 * b0) + (a1 * a1 + b1 * b1);
+    if (zb > 1e-8) {
+      const double c0 = a0 * a0 + b0 * b0, c1 = a1 * a1 + b1 * b1;
+      const double d0 = atan2(b0, a0), d1 = atan2(b1, a1);
+      const double eb = c1 / c0 - 1;
+      double err = eb * eb;
+      err /= n;
+      if (err < 1e-12 && c0 > 0.01) {
+        const double slope = (d1 - d0) * 1.0;
+        int freq = (int)rint(N_OVER_PI2 * slope) + n;
+        freq = (int)((unsigned)freq & n1);
+        const unsigned hashed = (unsigned)(long long)rint(freq * BUCKETS_OVER_N) & B1m;
+        if (hashed == (unsigned)bk) {
+          const double ph = d0 - PI2_A_OFFSET_OVER_N * freq;
+          const double mag = sqrt(c0);
+          double sn, cs;
+          sincos(ph, &sn, &cs);
+          const double mr = mag * cs, mi = mag * sn;
+          const unsigned dist =
+              (hashed * (unsigned)N_OVER_BUCKETS - (unsigned)freq + (unsigned)n) & n1;
+          const double2 F = win_at(win, half, dist, n);
+          const double den = F.x * F.x + F.y * F.y;
+          val = make_double2((mr * F.x + mi * F.y) / den, (mi * F.x - mr * F.y) / den);
+          out = (int)(((unsigned)(timesmod(freq, a_, n) - b_ + n)) & n1);
+        }
+      }
+    }
+    c.cf[bk] = out;
+    c.cv[bk] = val;
+  }
+  __threadfence_block();
+  __syncthreads();
+  return compact(c, B, sh);
+}
+
+// estimate_freq_mansour_loops2 (cf3 642-776)
+__device__ int est_mansour(const Ctx &c, int *sh)
+{
+  const int n = c.n, W = c.W;
+  const double PI2 = 2 * M_PI;
+  const double N_OVER_PI2 = n / PI2;
+  const double PI2_OVER_N = PI2 / n;
+  const unsigned FREQ_MASK = ((unsigned)n - 1) & ~((unsigned)W - 1);
+  const double NORM = 1. / W;
+  const double NORM2 = NORM * NORM;
+  for (int bk = threadIdx.x; bk < W; bk += blockDim.x) {
+    int out = -1;
+    double2 val = make_double2(0, 0);
+    const double2 s0 = c.MS[2 * bk], s1 = c.MS[2 * bk + 1];
+    const double a0 = s0.x, b0 = s0.y, a1 = s1.x, b1 = s1.y;
+    const double zb = (a0 * a0 + b0 * b0) * NORM2 + (a1 * a1 + b1 * b1) * NORM2;
+    if (zb > 1e-8) {
+      const double c0 = (a0 * a0 + b0 * b0) * NORM2;
+      const double c1 = (a1 * a1 + b1 * b1) * NORM2;
+      const double d0 = atan2(b0 * NORM, a0 * NORM), d1 = atan2(b1 * NORM, a1 * NORM);
+      double median_abs = c0;
+      const double inv = 1. / median_abs;
+      const double bb = (c1 * inv) - 1;
+      const double err = bb * bb;
+      if (err < n * 1e-10 && median_abs > 0.01) {
+        const double slope = d1 - d0;
+        const int f1 = (int)rint(slope * N_OVER_PI2);
+        const unsigned f2 = (unsigned)f1 & FREQ_MASK;
+        const int f3 = (int)(f2 | (unsigned)bk);
+        const int fo = wmul(f3, c.g0);
+        const double ph = d0 - PI2_OVER_N * fo;
+        median_abs = sqrt(median_abs);
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        out = f3;
+        val = make_double2(median_abs * cs, median_abs * sn);
+      }
+    }
+    c.cf[bk] = out;
+    c.cv[bk] = val;
+  }
+  __threadfence_block();
+  __syncthreads();
+  return compact(c, W, sh);
+}
+
+// Adds the F found (ef, ev) to the map (new keys appended in found order).
+// Returns false when the map would overflow.
+__device__ bool map_add_found(const Ctx &c, int F, int *sh, int *s_size)
+{
+  const int T = blockDim.x;
+  const int C = (F + T - 1) / T;
+  const int lo = threadIdx.x * C, hi = min(F, lo + C);
+  int nnew = 0;
+  for (int i = lo; i < hi; i++) {
+    const int e = map_find(c, c.ef[i]);
+    c.remi[i * 3] = e;  // remember (reused as scratch before removals)
+    nnew += e < 0;
+  }
+  int total;
+  int pos = block_exclusive_scan(nnew, sh, total);
+  const int base = *s_size;
+  if (base + total > c.a->Mcap) return false;
+  for (int i = lo; i < hi; i++) {
+    int e = c.remi[i * 3];
+    if (e < 0) {
+      e = base + pos++;
+      c.keys[e] = c.ef[i];
+      c.vals[e] = make_double2(0.0, 0.0);
+      map_insert_slot(c, c.ef[i], e);
+    }
+    // ans[f] = ans[f] + v (distinct keys within one pass)
+    c.vals[e] = cadd(c.vals[e], c.ev[i]);
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (threadIdx.x == 0) *s_size = base + total;
+  __syncthreads();
+  return true;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a)
+{
+  extern __shared__ double2 lds[];
+  __shared__ int sh[16];
+  __shared__ int s_size, s_flag, s_cnt[3];
+  const int s = blockIdx.x;
+  Ctx c;
+  c.a = &a;
+  c.s = s;
+  c.n = a.n;
+  c.W = a.W;
+  c.B1 = a.B1;
+  c.B2 = a.B2;
+  const int *prm = a.params + (int64_t)s * 6;
+  c.aa = prm[0];
+  c.ai = prm[1];
+  c.b = prm[2];
+  c.shift = prm[3];
+  c.g0 = prm[4];
+  c.gG = prm[5];
+  c.GS0 = lds;
+  c.GS1 = lds + a.B1;
+  c.GP0 = lds + 2 * a.B1;
+  c.GP1 = lds + 2 * a.B1 + a.B2;
+  c.MS = a.ms + (int64_t)s * 2 * a.W;
+  c.keys = a.map_keys + (int64_t)s * a.Mcap;
+  c.vals = a.map_vals + (int64_t)s * a.Mcap;
+  c.hidx = a.map_hidx + (int64_t)s * a.Hcap;
+  c.cf = a.cf + (int64_t)s * a.Fcap;
+  c.cv = a.cv + (int64_t)s * a.Fcap;
+  c.ef = a.ef + (int64_t)s * a.Fcap;
+  c.ev = a.ev + (int64_t)s * a.Fcap;
+  c.rem = a.rem + (int64_t)s * a.Rcap * 14;
+  c.remi = a.remi + (int64_t)s * a.Rcap * 3;
+  const int n = a.n, W = a.W;
+  const unsigned mask = (unsigned)n - 1;
+  const double2 *x = a.x[s];
+  for (int i = threadIdx.x; i < a.Hcap; i += blockDim.x) c.hidx[i] = -1;
+  if (threadIdx.x == 0) {
+    s_size = 0;
+    s_flag = 0;
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  // ---- Mansour filter (cf3 96-127): two aliased sub-samplings, W-pt FFTs
+  {
+    const int sigma = n / W, logW = ilog2v(W);
+    for (int l = 0; l < 2; l++) {
+      for (int i = threadIdx.x; i < W; i += blockDim.x)
+        lds[bitrev((unsigned)i, logW)] =
+            x[((unsigned)c.g0 + (unsigned)l + (unsigned)i * (unsigned)sigma) & mask];
+      __syncthreads();
+      lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
+      for (int i = threadIdx.x; i < W; i += blockDim.x) c.MS[2 * i + l] = lds[i];
+      __syncthreads();
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  int F = est_mansour(c, sh);
+  if (!map_add_found(c, F, sh, &s_size)) {
+    if (threadIdx.x == 0) s_flag |= 2;
+  }
+  for (int i = threadIdx.x; i < F; i += blockDim.x) {
+    const int h = c.ef[i] % W;
+    c.MS[2 * h] = make_double2(0, 0);
+    c.MS[2 * h + 1] = make_double2(0, 0);
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (F == a.k) {  // cf3 859-860: the reference returns with out untouched
+    if (threadIdx.x == 0) {
+      a.count[s] = 0;
+      a.status[s] = 1;
+    }
+    return;
+  }
+
+  // ---- Gauss filter 1 (cf3 130-212): contiguous taps folded into B1
+  {
+    const int B = a.B1, chunks = a.w1 / B;
+    for (int r = threadIdx.x; r < B; r += blockDim.x) {
+      double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
+      for (int ch = 0; ch < chunks; ch++) {
+        const int t = ch * B + r;
+        const double2 tp = a.taps1[t];
+        const double2 x0 = x[((unsigned)c.gG + (unsigned)t) & mask];
+        const double2 x1 = x[((unsigned)c.gG + 1u + (unsigned)t) & mask];
+        s0.x = s0.x + (tp.x * x0.x - tp.y * x0.y);
+        s0.y = s0.y + (tp.y * x0.x + tp.x * x0.y);
+        s1.x = s1.x + (tp.x * x1.x - tp.y * x1.y);
+        s1.y = s1.y + (tp.y * x1.x + tp.x * x1.y);
+      }
+      c.GS0[r] = s0;
+      c.GS1[r] = s1;
+    }
+    __syncthreads();
+    fft_pair(c.GS0, c.GS1, B, a);
+  }
+  subtract(c, c.keys, c.vals, s_size, 1);  // the map so far, insertion order
+  F = est_gauss(c, c.GS0, c.GS1, a.B1, 1, 0, a.win1, a.win1_half, sh);
+  if (!map_add_found(c, F, sh, &s_size) && threadIdx.x == 0) s_flag |= 2;
+  __syncthreads();
+  subtract(c, c.ef, c.ev, F, 1 | 4);
+
+  // ---- Gauss filter 2 (cf3 215-297): permuted samples times e^{2 pi i b t/n}
+  {
+    // permute_fourier (cf3 66-87) forms the phase of sample i as
+    // shift * step^i by repeated complex multiplication; one lane replays
+    // that chain so the samples carry the reference's rounding.
+    double2 *phase = a.phase + (int64_t)s * a.Pcap;
+    if (threadIdx.x == 0) {
+      double s0, c0, s1, c1;
+      sincos(2 * M_PI * c.gG * c.b / n, &s0, &c0);
+      sincos(2 * M_PI * c.ai * c.b / n, &s1, &c1);
+      double2 sh = make_double2(c0, s0);
+      const double2 st = make_double2(c1, s1);
+      for (int i = 0; i <= a.w2; i++) {
+        phase[i] = sh;
+        sh = make_double2(sh.x * st.x - sh.y * st.y, sh.x * st.y + sh.y * st.x);
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    const int B = a.B2, chunks = a.w2 / B;
+    for (int r = threadIdx.x; r < B; r += blockDim.x) {
+      double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
+      for (int ch = 0; ch < chunks; ch++) {
+        const int t = ch * B + r;
+        const double2 tp = a.taps2[t];
+        double2 p[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          // sample (gG + (t+q) ai) mod n (cf3 78-85)
+          const unsigned idx =
+              (unsigned)(((long long)c.gG + (long long)(t + q) * c.ai) % n);
+          const double2 xv = x[idx], sh = phase[t + q];
+          p[q] = make_double2(xv.x * sh.x - xv.y * sh.y, xv.x * sh.y + xv.y * sh.x);
+        }
+        s0.x = s0.x + (tp.x * p[0].x - tp.y * p[0].y);
+        s0.y = s0.y + (tp.y * p[0].x + tp.x * p[0].y);
+        s1.x = s1.x + (tp.x * p[1].x - tp.y * p[1].y);
+        s1.y = s1.y + (tp.y * p[1].x + tp.x * p[1].y);
+      }
+      c.GP0[r] = s0;
+      c.GP1[r] = s1;
+    }
+    __syncthreads();
+    fft_pair(c.GP0, c.GP1, B, a);
+  }
+  subtract(c, c.keys, c.vals, s_size, 2);
+  F = est_gauss(c, c.GP0, c.GP1, a.B2, c.aa, c.b, a.win2, a.win2_half, sh);
+  if (!map_add_found(c, F, sh, &s_size) && threadIdx.x == 0) s_flag |= 2;
+  __syncthreads();
+  subtract(c, c.ef, c.ev, F, 7);
+
+  // ---- round robin until the three bucket counts repeat (cf3 997-1076)
+  int prev[3] = {0, 0, 0};
+  int pass = 0;
+  for (; pass < a.max_passes; pass++) {
+    const int sel = pass % 3;
+    if (sel == 0) F = est_mansour(c, sh);
+    else if (sel == 1) F = est_gauss(c, c.GS0, c.GS1, a.B1, 1, 0, a.win1, a.win1_half, sh);
+    else F = est_gauss(c, c.GP0, c.GP1, a.B2, c.aa, c.b, a.win2, a.win2_half, sh);
+    if (!map_add_found(c, F, sh, &s_size)) {
+      if (threadIdx.x == 0) s_flag |= 2;
+      __syncthreads();
+      break;
+    }
+    subtract(c, c.ef, c.ev, F, 7);
+    if (sel == 2) {
+      if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+      __syncthreads();
+      int c1 = 0, c2 = 0, cm = 0;
+      for (int j = threadIdx.x; j < a.B1; j += blockDim.x) {
+        // the reference counts elements 0..B1-1 of its interleaved array
+        const double2 z = (j & 1) ? c.GS1[j >> 1] : c.GS0[j >> 1];
+        c1 += z.x * z.x + z.y * z.y > 1e-6;
+      }
+      for (int j = threadIdx.x; j < a.B2; j += blockDim.x) {
+        const double2 z = (j & 1) ? c.GP1[j >> 1] : c.GP0[j >> 1];
+        c2 += z.x * z.x + z.y * z.y > 1e-6;
+      }
+      for (int j = threadIdx.x; j < W; j += blockDim.x) {
+        const double2 z = c.MS[2 * j];
+        const double re = z.x / W, im = z.y / W;
+        cm += re * re + im * im > 1e-6;
+      }
+      atomicAdd(&s_cnt[0], cm);
+      atomicAdd(&s_cnt[1], c1);
+      atomicAdd(&s_cnt[2], c2);
+      __syncthreads();
+      const int now[3] = {s_cnt[0], s_cnt[1], s_cnt[2]};
+      __syncthreads();
+      if (now[0] == prev[0] && now[1] == prev[1] && now[2] == prev[2]) break;
+      prev[0] = now[0];
+      prev[1] = now[1];
+      prev[2] = now[2];
+    }
+  }
+  if (pass >= a.max_passes && threadIdx.x == 0) s_flag |= 4;
+  __syncthreads();
+
+  // ---- result: the map sorted by frequency
+  const int M = s_size;
+  int P = 1;
+  while (P < M) P <<= 1;
+  uint64_t *srt = reinterpret_cast<uint64_t *>(lds);
+  const bool sortable = (size_t)P * 8 <= a.lds_bytes;
+  if (sortable) {
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+      srt[i] = i < M ? (((uint64_t)(unsigned)c.keys[i] << 32) | (unsigned)i) : ~0ull;
+    __syncthreads();
+    for (int k2 = 2; k2 <= P; k2 <<= 1) {
+      for (int j = k2 >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+          const int p = i ^ j;
+          if (p > i) {
+            const uint64_t u = srt[i], v = srt[p];
+            const bool up = (i & k2) == 0;
+            if ((u > v) == up) {
+              srt[i] = v;
+              srt[p] = u;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  for (int t = threadIdx.x; t < M; t += blockDim.x) {
+    const int e = sortable ? (int)(srt[t] & 0xffffffffu) : t;
+    const int key = c.keys[e];
+    const double2 v = c.vals[e];
+    if (a.o_freq && t < a.o_cap) {
+      int32_t *of = a.o_freq[s];
+      double2 *ov = a.o_val[s];
+      if (of) of[t] = key;
+      if (ov) ov[t] = v;
+    }
+    if (a.o_dense && a.o_dense[s]) a.o_dense[s][key] = v;
+  }
+  if (threadIdx.x == 0) {
+    a.count[s] = M;
+    a.status[s] = s_flag;
+  }
+}
+
+int v3(const V3Launch &a, hipStream_t st)
+{
+  if (a.lds_bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_v3),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+  hipLaunchKernelGGL(k_v3, dim3(a.S), dim3(kV3Block), a.lds_bytes, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("v3: ") + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+}  // namespace k
+}  // namespace sfa
