@@ -159,6 +159,14 @@ int sfft_plan_stage_times(const sfft_plan *plan, double *ms6,
 int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
                            int64_t count);
 
+/* The plan's filters (which: 0 = location / first Gauss filter, 1 =
+ * estimation / second Gauss filter): its w taps into time (capacity
+ * time_cap) and its response at offsets -half..half into resp (capacity
+ * resp_cap, resp[d + half] = response at frequency d mod n; the only
+ * offsets the exec reads).  Returns w, or -1. */
+int sfft_plan_filter(const sfft_plan *plan, int which, complex_t *time, int time_cap,
+                     complex_t *resp, int resp_cap, int *half);
+
 /* Host-side filter design of the library (src/filters.cc:70-160), for
  * tests: w taps into time (capacity time_cap) and the n-point response into
  * freq (may be NULL).  Returns w, or -1 when time_cap < w. */

@@ -17,7 +17,7 @@ EXPORTS = [
     "sfft_plan_set_stream", "sfft_plan_set_reference_conjugate",
     "sfft_plan_memory", "sfft_last_error", "sfft_plan_params",
     "sfft_derive_params", "sfft_design_filter", "sfft_debug_buckets",
-    "sfft_plan_enable_timing", "sfft_plan_stage_times",
+    "sfft_plan_enable_timing", "sfft_plan_stage_times", "sfft_plan_filter",
 ]
 
 SFFT_VERSION_1, SFFT_VERSION_2, SFFT_VERSION_3 = 0, 1, 2
@@ -74,6 +74,7 @@ def load():
     L.sfft_debug_buckets.argtypes = [PP, C.c_int, P, C.c_int64]
     L.sfft_debug_buckets.restype = C.c_int64
     L.sfft_plan_enable_timing.argtypes = [PP, C.c_int]
+    L.sfft_plan_filter.argtypes = [PP, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_int)]
     L.sfft_plan_stage_times.argtypes = [PP, P, P]
     _lib = L
     return L

@@ -413,6 +413,30 @@ int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
   return m;
 }
 
+int sfft_plan_filter(const sfft_plan *plan, int which, complex_t *time, int time_cap,
+                     complex_t *resp, int resp_cap, int *half)
+{
+  Plan *P = get(plan);
+  if (!P || which < 0 || which > 1) return -1;
+  const std::vector<cd> &t = P->f_time[which], &r = P->f_resp[which];
+  if (half) *half = P->resp_half[which];
+  if (time) {
+    if (time_cap < (int)t.size()) {
+      set_error("time buffer too small");
+      return -1;
+    }
+    for (size_t i = 0; i < t.size(); i++) time[i] = t[i];
+  }
+  if (resp) {
+    if (resp_cap < (int)r.size()) {
+      set_error("response buffer too small");
+      return -1;
+    }
+    for (size_t i = 0; i < r.size(); i++) resp[i] = r[i];
+  }
+  return (int)t.size();
+}
+
 int sfft_design_filter(double lobefrac, double tolerance, int n, int b,
                        complex_t *time, int time_cap, complex_t *freq)
 {

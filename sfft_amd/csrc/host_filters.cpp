@@ -94,9 +94,51 @@ void host_fft(cd *x, int n, bool backward)
     fft_any(x, n, backward);
 }
 
+namespace {
+
+using FftFn = bool (*)(cd *x, int n, bool backward);
+
+bool host_fft_fn(cd *x, int n, bool backward)
+{
+  host_fft(x, n, backward);
+  return true;
+}
+
+// n-point DFT on the GPU (Stockham passes), plan time only.
+bool gpu_fft_fn(cd *x, int n, bool backward)
+{
+  double2 *d = nullptr, *t = nullptr;
+  const size_t bytes = (size_t)n * sizeof(double2);
+  bool ok = hipMalloc(&d, bytes) == hipSuccess && hipMalloc(&t, bytes) == hipSuccess &&
+            hipMemcpy(d, x, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+            k::batch_fft64(d, t, n, 1, 1, n, n, backward, nullptr, 0, nullptr) == 0 &&
+            hipMemcpy(x, d, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  if (d) (void)hipFree(d);
+  if (t) (void)hipFree(t);
+  if (!ok) set_error("plan-time FFT on the GPU failed");
+  return ok;
+}
+
+bool design(double lobefrac, double tolerance, int n, int b, FftFn nfft, HostFilter &f);
+
+}  // namespace
+
 HostFilter design_filter(double lobefrac, double tolerance, int n, int b)
 {
   HostFilter f;
+  design(lobefrac, tolerance, n, b, host_fft_fn, f);
+  return f;
+}
+
+bool design_filter_gpu(double lobefrac, double tolerance, int n, int b, HostFilter &f)
+{
+  return design(lobefrac, tolerance, n, b, gpu_fft_fn, f);
+}
+
+namespace {
+
+bool design(double lobefrac, double tolerance, int n, int b, FftFn nfft, HostFilter &f)
+{
   // Dolph-Chebyshev window: w-point DFT of sampled T_{w-1}, centred, real
   // part (src/filters.cc:70-86).
   const int w = dolph_chebyshev_width(lobefrac, tolerance);
@@ -115,7 +157,7 @@ HostFilter design_filter(double lobefrac, double tolerance, int n, int b)
   std::vector<cd> g(n, cd(0, 0));
   std::copy(centred.begin() + w / 2, centred.end(), g.begin());
   std::copy(centred.begin(), centred.begin() + w / 2, g.end() - w / 2);
-  host_fft(g.data(), n, false);
+  if (!nfft(g.data(), n, false)) return false;
   f.freq.assign(n, cd(0, 0));
   cd run(0, 0);
   for (int i = 0; i < b; i++) run += g[i];
@@ -135,10 +177,11 @@ HostFilter design_filter(double lobefrac, double tolerance, int n, int b)
     rot *= step;
   }
   std::vector<cd> t(f.freq);
-  host_fft(t.data(), n, true);
+  if (!nfft(t.data(), n, true)) return false;
   f.time.assign(t.begin(), t.begin() + w);
   for (int i = 0; i < w; i++) f.time[i] /= (double)n;
-  return f;
+  return true;
 }
 
+}  // namespace
 }  // namespace sfa

@@ -168,10 +168,12 @@ std::string check_v12(const V12Params &p)
   if (p.comb && (p.W_comb < num + 1 || p.n % p.W_comb != 0))
     return "W_Comb < 2k+1: the reference's comb selection cannot run";
   if (p.loops() > kMaxLoops) return "more than 32 loops is not supported";
-  if (p.B_loc > kMaxLdsFft64 || p.B_est > kMaxLdsFft64)
-    return "bucket FFT above 8192 points is not supported yet";
-  if (p.comb && p.W_comb > kMaxLdsFft64)
-    return "comb FFT above 8192 points is not supported yet";
+  // v1 vote bitmap: loc_loops * B_loc bits in 64 KiB of LDS
+  if ((int64_t)p.loc_loops * ((p.B_loc + 31) / 32) * 4 > 65536)
+    return "loc_loops * B_loc above 2^19 is not supported";
+  if (p.comb && p.W_comb > (1 << 22)) return "comb width above 2^22 is not supported";
+  if (p.B_loc > (1 << 22) || p.B_est > (1 << 22))
+    return "bucket count above 2^22 is not supported";
   if (p.w_loc < 1 || p.w_est < 1) return "empty filter";
   if (p.w_loc > p.n || p.w_est > p.n) return "filter longer than n";
   return "";

@@ -162,12 +162,20 @@ int64_t Plan::plan_bytes() const
 int64_t Plan::bytes_per_signal() const
 {
   if (version == SFFT_VERSION_3)
-    return 32LL * v3.W_Man + (int64_t)v3_Mcap * 28 + (int64_t)v3_Rcap * 236 +
+    return (v3.W_Man > kMaxLdsFft64 ? 96LL : 32LL) * v3.W_Man + (int64_t)v3_Mcap * 28 + (int64_t)v3_Rcap * 236 +
            (int64_t)v3_Fcap * 40 + 16LL * (v3.w_g2 + 2) + 512;
   const V12Params &p = v12;
-  return p.x_samp_size() * 16 + (int64_t)p.comb_loops * p.B_thresh * 4 +
-         (int64_t)p.loc_loops * p.B_thresh * 4 + output_capacity() * (4 + 16 + 4) +
-         (2 * p.loops() + p.comb_loops + 8) * 4;
+  const bool fp32 = precision == SFFT_PRECISION_FP32_REFINE;
+  const bool need64 = !fp32 || !p.comb;
+  const int64_t xs = p.x_samp_size();
+  const int Bm = std::max(p.B_loc, p.B_est);
+  int64_t b = (need64 ? xs * 16 : 0) + (fp32 ? xs * 8 : 0);
+  if (need64 && Bm > kMaxLdsFft64) b += xs * 16;
+  if (fp32 && Bm > 2 * kMaxLdsFft64) b += xs * 8;
+  if (p.comb && p.W_comb > k::kMaxLdsComb) b += (int64_t)p.comb_loops * p.W_comb * 40;
+  if (!p.comb && p.B_loc > k::kMaxLdsSelect) b += (int64_t)p.loc_loops * p.B_loc * 8;
+  return b + (int64_t)p.comb_loops * p.B_thresh * 4 + (int64_t)p.loc_loops * p.B_thresh * 4 +
+         output_capacity() * (4 + 16 + 4) + (2 * p.loops() + p.comb_loops + 8) * 4;
 }
 
 // Filter response window at signed offsets -half..half (freq[d mod n]);
@@ -180,6 +188,17 @@ static std::vector<double2> centred_window(const std::vector<cd> &freq, int n, i
     w[d + half] = make_double2(v.real(), v.imag());
   }
   return w;
+}
+
+// Host copy of a filter for sfft_plan_filter: taps and the response at
+// offsets -half..half.
+static void keep_filter(Plan *P, int which, const HostFilter &f, int n, int half)
+{
+  P->f_time[which] = f.time;
+  P->f_resp[which].resize(2 * (size_t)half + 1);
+  for (int d = -half; d <= half; d++)
+    P->f_resp[which][d + half] = f.freq[(size_t)(((int64_t)d % n + n) % n)];
+  P->resp_half[which] = half;
 }
 
 static std::vector<cd> twiddles(int nmax)
@@ -208,21 +227,27 @@ static int make_v3_tables(Plan *P)
 {
   const V3Params &q = P->v3;
   const int n = q.n;
-  HostFilter f1 = design_filter(0.5 / q.BB1, 1e-8, n, q.b_g1);
-  HostFilter f2 = design_filter(0.5 / q.BB2, 1e-8, n, q.b_g2);
+  HostFilter f1, f2;
+  if (!design_filter_gpu(0.5 / q.BB1, 1e-8, n, q.b_g1, f1) ||
+      !design_filter_gpu(0.5 / q.BB2, 1e-8, n, q.b_g2, f2))
+    return -1;
   P->v3_half1 = 3 * (n / q.B_g1) / 2;
   P->v3_half2 = 3 * (n / q.B_g2) / 2;
+  keep_filter(P, 0, f1, n, P->v3_half1);
+  keep_filter(P, 1, f2, n, P->v3_half2);
   P->tw_nmax = std::max(q.W_Man, std::max(q.B_g1, q.B_g2));
   // map of located frequencies: room for several passes' spurious finds
   P->v3_Mcap = pow2_ceil(8LL * q.k + q.W_Man + 2LL * q.B_g1 + 2LL * q.B_g2 + 64);
   P->v3_Hcap = 2 * P->v3_Mcap;
   P->v3_Fcap = std::max(q.W_Man, std::max(q.B_g1, q.B_g2));
   P->v3_Rcap = std::max(P->v3_Mcap, P->v3_Fcap);
-  P->v3_lds = std::max<size_t>((size_t)q.W_Man * 16,
+  // the Mansour FFTs run in LDS up to kMaxLdsFft64 points, else before the
+  // persistent kernel as batched radix-2 passes
+  P->v3_lds = std::max<size_t>(q.W_Man <= kMaxLdsFft64 ? (size_t)q.W_Man * 16 : 0,
                                (size_t)(2 * q.B_g1 + 2 * q.B_g2) * 16);
   P->v3_lds = std::max<size_t>(P->v3_lds, std::min<size_t>((size_t)P->v3_Mcap * 8, 131072));
   if (P->v3_lds > 160 * 1024) {
-    set_error("v3 Mansour filter above 8192 buckets is not supported yet");
+    set_error("v3 Gauss filters above 2560 buckets are not supported");
     return -1;
   }
   const bool ok = upload(P->v3_time1, to_d2(f1.time, f1.time.size())) &&
@@ -274,8 +299,14 @@ Plan *make_plan(int n, int k, sfft_version v)
     return nullptr;
   }
   const V12Params &p = P->v12;
-  HostFilter fl = design_filter(0.5 / p.BB_loc, p.tol_loc, p.n, p.b_loc);
-  HostFilter fe = design_filter(0.5 / p.BB_est, p.tol_est, p.n, p.b_est);
+  HostFilter fl, fe;
+  if (!design_filter_gpu(0.5 / p.BB_loc, p.tol_loc, p.n, p.b_loc, fl) ||
+      !design_filter_gpu(0.5 / p.BB_est, p.tol_est, p.n, p.b_est, fe)) {
+    free_plan(P);
+    return nullptr;
+  }
+  keep_filter(P, 0, fl, p.n, (p.n / p.B_loc) / 2);
+  keep_filter(P, 1, fe, p.n, (p.n / p.B_est) / 2);
   int nmax = std::max(p.B_loc, p.B_est);
   if (p.comb) nmax = std::max(nmax, p.W_comb);
   const std::vector<cd> tw = twiddles(nmax);
@@ -307,6 +338,7 @@ void free_plan(Plan *P)
                     &P->ws_params, &P->ws_xsamp, &P->ws_xsamp32, &P->ws_approved,
                     &P->ws_numcomb, &P->ws_J, &P->ws_hits,
                     &P->ws_ptrs, &P->ws_out_freq, &P->ws_out_val,
+                    &P->ws_comb, &P->ws_mag, &P->ws_tmp64, &P->ws_tmp32,
                     &P->ws_count, &P->ws_stage, &P->ws_v3};
   for (DevBuf *b : bufs) b->release();
   if (P->ev_made)
@@ -372,8 +404,10 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
                b_rem = al(Sz * P->v3_Rcap * 14 * 16), b_remi = al(Sz * P->v3_Rcap * 3 * 4);
   const int Pcap = q.w_g2 + 2;
   const size_t b_ph = al(Sz * Pcap * 16);
-  const size_t total =
-      b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi + b_ph;
+  const bool big_ms = q.W_Man > kMaxLdsFft64;
+  const size_t b_pre = big_ms ? al(Sz * 2 * q.W_Man * 16) : 0;
+  const size_t total = b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi +
+                       b_ph + 2 * b_pre;
   if (!P->ws_params.ensure(param_bytes) || !P->ws_ptrs.ensure(4 * tab_bytes) ||
       !P->ws_v3.ensure(total) || !P->ws_count.ensure(cnt_bytes) ||
       !P->ws_v3_status.ensure((size_t)S * 4))
@@ -406,6 +440,10 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   a.tw_nmax = P->tw_nmax;
   a.ms = reinterpret_cast<double2 *>(w);
   w += b_ms;
+  a.ms_pre = big_ms ? reinterpret_cast<double2 *>(w) : nullptr;
+  w += b_pre;
+  a.ms_tmp = big_ms ? reinterpret_cast<double2 *>(w) : nullptr;
+  w += b_pre;
   a.map_keys = reinterpret_cast<int *>(w);
   w += b_keys;
   a.map_vals = reinterpret_cast<double2 *>(w);
@@ -527,6 +565,19 @@ int exec_batch(Plan *P, BatchIO &io)
       !P->ws_hits.ensure((size_t)S * (p.comb ? 1 : cap) * 4) ||
       !P->ws_count.ensure((size_t)S * 8))
     return -1;
+  // scratch for the sizes above the LDS limits
+  const bool big_comb = p.comb && p.W_comb > k::kMaxLdsComb;
+  const bool big_sel = !p.comb && p.B_loc > k::kMaxLdsSelect;
+  const bool big64 = need64 && std::max(p.B_loc, p.B_est) > kMaxLdsFft64;
+  const bool big32 = fp32 && std::max(p.B_loc, p.B_est) > 2 * kMaxLdsFft64;
+  const size_t comb_elems = big_comb ? (size_t)S * cl * p.W_comb : 0;
+  const size_t mag_elems =
+      std::max(comb_elems, big_sel ? (size_t)S * p.loc_loops * p.B_loc : (size_t)0);
+  if ((big_comb && !P->ws_comb.ensure(comb_elems * 2 * sizeof(double2))) ||
+      (mag_elems && !P->ws_mag.ensure(mag_elems * sizeof(double))) ||
+      (big64 && !P->ws_tmp64.ensure((size_t)S * xs * 16)) ||
+      (big32 && !P->ws_tmp32.ensure((size_t)S * xs * 8)))
+    return -1;
   hipStream_t st = P->stream;
   HIP_OK(hipMemcpyAsync(P->ws_params.p, hp, (size_t)S * stride * sizeof(int),
                         hipMemcpyHostToDevice, st));
@@ -575,7 +626,13 @@ int exec_batch(Plan *P, BatchIO &io)
   a.o_dense = want_dense ? reinterpret_cast<double2 *const *>(dtab + 3 * tab_bytes) : nullptr;
   a.o_cap = io.cap;
   a.est_cap = cap;
-  a.fold_mode = P->fold_mode;
+  a.comb_buf = big_comb ? P->ws_comb.as<double2>() : nullptr;
+  a.comb_tmp = big_comb ? P->ws_comb.as<double2>() + comb_elems : nullptr;
+  a.mag_buf = mag_elems ? P->ws_mag.as<double>() : nullptr;
+  a.tmp64 = big64 ? P->ws_tmp64.as<double2>() : nullptr;
+  a.tmp32 = big32 ? P->ws_tmp32.as<float2>() : nullptr;
+  // the fused fold + FFT keeps a whole bucket array in LDS
+  a.fold_mode = std::max(p.B_loc, p.B_est) > kMaxLdsFft64 ? 1 : P->fold_mode;
   a.stream_blocks = P->stream_blocks;
   if (a.stream_blocks <= 0) {
     int dev = 0, cus = 256;

@@ -65,6 +65,10 @@ struct HostFilter {
 };
 // Dolph-Chebyshev window flattened over b bins (src/filters.cc:70-160).
 HostFilter design_filter(double lobefrac, double tolerance, int n, int b);
+// The same design with its two n-point DFTs on the GPU (plan time); the
+// running sum, normalisation and iterated phase ramp stay on the host.
+// Returns false (set_error) on a HIP failure.
+bool design_filter_gpu(double lobefrac, double tolerance, int n, int b, HostFilter &f);
 
 // ---------------------------------------------------------------- RNG
 // The reference's random() / drand48() draws, from the process-global glibc
@@ -111,6 +115,11 @@ struct Plan {
   DevBuf tw64, tw32;              // e^{-2 pi i j / Nmax}, j < Nmax/2
   int tw_nmax = 0;
 
+  // host copies of the two filters: taps, and the response at signed offsets
+  // -resp_half..resp_half (resp[d + resp_half] = freq[d mod n])
+  std::vector<cd> f_time[2], f_resp[2];
+  int resp_half[2] = {0, 0};
+
   // v3 device tables
   DevBuf v3_time1, v3_time2;  // double2[w_g*]
   DevBuf v3_freq1, v3_freq2;  // double2[3n/B + 1]: responses at offsets -3n/2B..3n/2B
@@ -123,6 +132,9 @@ struct Plan {
   // per-exec workspace (grow-only)
   DevBuf ws_params, ws_xsamp, ws_xsamp32, ws_approved, ws_numcomb, ws_J,
       ws_hits, ws_ptrs, ws_out_freq, ws_out_val, ws_count, ws_stage, ws_v3;
+  // scratch of the sizes above the LDS limits (comb FFT, v1 selection,
+  // bucket FFT passes)
+  DevBuf ws_comb, ws_mag, ws_tmp64, ws_tmp32;
   int last_S = 0;  // signals in the last batch (for diagnostics)
   std::vector<char> host_stage;
   void *pinned = nullptr;
@@ -193,7 +205,16 @@ struct V12Launch {
   int fold_mode;            // 0: fused fold+FFT per (signal, loop) block;
                             // 1: signal-major streamed fold, then bucket FFTs
   int stream_blocks;        // resident blocks of the streamed fold
+  // scratch for transforms and selections that do not fit in LDS (null
+  // when unused)
+  double2 *comb_buf, *comb_tmp;  // S x comb_loops x W sub-sampled comb signals
+  double *mag_buf;               // |.|^2 of comb or v1 selection arrays
+  double2 *tmp64;                // S x xs: radix-2 pass copy of xs64
+  float2 *tmp32;                 // S x xs: radix-2 pass copy of xs32
 };
+// Largest comb transform / selection array kept in LDS.
+constexpr int kMaxLdsComb = 8192;
+constexpr int kMaxLdsSelect = 16384;
 struct V3Launch {
   int n, S, k;
   int W, B1, B2, w1, w2;
@@ -206,6 +227,8 @@ struct V3Launch {
   int tw_nmax;
   // per-signal workspace, strided by signal
   double2 *ms;               // 2W (interleaved loops)
+  double2 *ms_pre, *ms_tmp;  // W above the LDS size: S x 2 x W transforms
+                             // computed before k_v3 (else null)
   int *map_keys;             // Mcap (insertion order)
   double2 *map_vals;         // Mcap
   int *map_hidx;             // Hcap hash slots -> entry
@@ -228,6 +251,24 @@ struct V3Launch {
   size_t lds_bytes;
 };
 int v3(const V3Launch &a, hipStream_t st);
+
+// Batched power-of-two FFTs; transform t = (outer, inner) lives at
+// data + outer*so + inner*si.  In LDS for small forward sizes (tw: the
+// plan's table for tw_nmax >= N), else Stockham passes ping-ponging with tmp
+// (same layout); the result always ends in data.
+int batch_fft64(double2 *data, double2 *tmp, int N, int64_t nb_outer, int nb_inner,
+                int64_t so, int64_t si, bool backward, const double2 *tw, int tw_nmax,
+                hipStream_t st);
+int batch_fft32(float2 *data, float2 *tmp, int N, int64_t nb_outer, int nb_inner, int64_t so,
+                int64_t si, bool backward, const float2 *tw, int tw_nmax, hipStream_t st);
+
+// Forward radix-2 DIT FFTs of any power-of-two size with the plan's twiddle
+// table, bit-identical to the in-LDS form; tmp (same layout as data) is
+// needed when N exceeds kMaxLdsFft64 (FP64) / 2*kMaxLdsFft64 (FP32).
+int radix2_fft64(double2 *data, double2 *tmp, int N, int64_t nb_outer, int nb_inner, int64_t so,
+                 int64_t si, const double2 *tw, int tw_nmax, hipStream_t st);
+int radix2_fft32(float2 *data, float2 *tmp, int N, int64_t nb_outer, int nb_inner, int64_t so,
+                 int64_t si, const float2 *tw, int tw_nmax, hipStream_t st);
 
 int comb(const V12Launch &a, hipStream_t st);
 // part 0: the fold (fused with the bucket FFTs when fold_mode == 0);

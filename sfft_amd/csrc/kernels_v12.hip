@@ -33,6 +33,11 @@ __device__ __forceinline__ int64_t loop_offset(int l, int Lloc, int Bl, int Be)
   return (int64_t)min(l, Lloc) * Bl + (int64_t)max(0, l - Lloc) * Be;
 }
 
+inline int64_t loop_offset_host(int l, int Lloc, int Bl, int Be)
+{
+  return (int64_t)std::min(l, Lloc) * Bl + (int64_t)std::max(0, l - Lloc) * Be;
+}
+
 // ---------------------------------------------------------------- comb
 // One block per (signal, comb loop).  LDS: W double2.
 __global__ void __launch_bounds__(kBlock) k_comb(V12Launch a)
@@ -72,6 +77,38 @@ __global__ void __launch_bounds__(kBlock) k_comb(V12Launch a)
   __syncthreads();
   int *out = a.approved + ((int64_t)s * cl + c) * a.num;
   block_top_num(mag, W, a.num, out, sh);
+  if (cl == 1 && threadIdx.x == 0) {
+    a.num_comb[s] = a.num;
+    a.count[s] = (int64_t)a.num * (a.n / W);
+  }
+}
+
+// W above the LDS limit: the strided sub-samples go to comb_buf, the W-point
+// FFTs run as LDS radix-2 runs plus global passes, then k_comb_select_global.
+__global__ void k_comb_gather(V12Launch a)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cl = a.comb_loops, W = a.W;
+  if (t >= (int64_t)a.S * cl * W) return;
+  const int i = (int)(t % W);
+  const int64_t sc = t / W;
+  const int s = (int)(sc / cl), c = (int)(sc % cl);
+  const int stride = 2 * a.L + max(cl, 1);
+  const int off = a.params[(int64_t)s * stride + 2 * a.L + c];
+  a.comb_buf[t] = a.x[s][(int64_t)off + (int64_t)i * (a.n / W)];
+}
+
+__global__ void __launch_bounds__(kBlock) k_comb_select_global(V12Launch a)
+{
+  __shared__ int sh[256 + 16];
+  const int cl = a.comb_loops, W = a.W;
+  const int s = blockIdx.x / cl, c = blockIdx.x % cl;
+  const double2 *z = a.comb_buf + (int64_t)blockIdx.x * W;
+  double *mag = a.mag_buf + (int64_t)blockIdx.x * W;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) mag[i] = z[i].x * z[i].x + z[i].y * z[i].y;
+  __threadfence_block();
+  __syncthreads();
+  block_top_num(mag, W, a.num, a.approved + ((int64_t)s * cl + c) * a.num, sh);
   if (cl == 1 && threadIdx.x == 0) {
     a.num_comb[s] = a.num;
     a.count[s] = (int64_t)a.num * (a.n / W);
@@ -253,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_fold_fft32(V12Launch a, int l0, int 
 // signal's 128-byte lines (20 loops gather ~20% of its samples) find them in
 // the MALL instead of HBM.  Each lane sums its bucket in ascending tap order
 // (identical arithmetic to the fused kernel); the B-point FFT follows in
-// k_bucket_fft.  Buckets are written pre-FFT into xs64/xs32.
+// the bucket FFT batches.  Buckets are written pre-FFT into xs64/xs32.
 constexpr int kFoldWaves = 4;  // wavefronts per block
 
 struct FoldItems {
@@ -367,34 +404,17 @@ __global__ void __launch_bounds__(64 * kFoldWaves) k_fold_stream(V12Launch a, Fo
   }
 }
 
-// In-place B-point FFT of every (signal, loop) bucket array of the range.
-template <bool FP32>
-__global__ void __launch_bounds__(kBlock) k_bucket_fft(V12Launch a, int l0, int nl)
-{
-  using T2 = typename std::conditional<FP32, float2, double2>::type;
-  extern __shared__ char smem_raw[];
-  T2 *buf = reinterpret_cast<T2 *>(smem_raw);
-  const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
-  const int B = l < a.Lloc ? a.B_loc : a.B_est;
-  const int logB = ilog2(B);
-  T2 *io;
-  if constexpr (FP32) io = a.xs32; else io = a.xs64;
-  io += (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
-  for (int m = threadIdx.x; m < B; m += blockDim.x) buf[bitrev(m, logB)] = io[m];
-  __syncthreads();
-  if constexpr (FP32) lds_fft(buf, logB, a.tw32, a.tw_nmax / B);
-  else lds_fft(buf, logB, a.tw64, a.tw_nmax / B);
-  for (int m = threadIdx.x; m < B; m += blockDim.x) io[m] = buf[m];
-}
-
 // ---------------------------------------------------------------- v1 votes
 // One block per (signal, location loop): top-num buckets by |.|^2.
 // LDS: B doubles.  v1 location loops are always FP64 (xs64): their top-2k
 // choice is made among leakage-level bucket powers FP32 cannot resolve.
+template <bool GLOBAL>
 __global__ void __launch_bounds__(kBlock) k_select(V12Launch a)
 {
-  extern __shared__ double mag[];
+  extern __shared__ double lds_mag[];
   __shared__ int sh[256 + 16];
+  // above the LDS size the magnitudes go to mag_buf
+  double *mag = GLOBAL ? a.mag_buf + (int64_t)blockIdx.x * a.B_loc : lds_mag;
   const int s = blockIdx.x / a.Lloc, l = blockIdx.x % a.Lloc;
   const int B = a.B_loc;
   const int64_t base = (int64_t)s * a.xs + (int64_t)l * a.B_loc;
@@ -403,6 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_select(V12Launch a)
     const double2 z = xs[i];
     mag[i] = z.x * z.x + z.y * z.y;
   }
+  if (GLOBAL) __threadfence_block();
   __syncthreads();
   block_top_num(mag, B, a.num, a.J + ((int64_t)s * a.Lloc + l) * a.num, sh);
 }
@@ -838,14 +859,36 @@ int launched(const char *what)
   return 0;
 }
 
+int launch_failed(const char *what)
+{
+  const hipError_t e = hipGetLastError();
+  set_error(std::string(what) + ": " + (e != hipSuccess ? hipGetErrorString(e)
+                                                         : "invalid transform shape"));
+  return -1;
+}
+
 }  // namespace
 
 int comb(const V12Launch &a, hipStream_t st)
 {
-  const size_t lds = (size_t)a.W * sizeof(double2);
-  allow_lds(k_comb, lds);
-  hipLaunchKernelGGL(k_comb, dim3(a.S * a.comb_loops), dim3(kBlock), lds, st, a);
-  if (launched("comb") != 0) return -1;
+  if (a.W <= kMaxLdsComb) {
+    const size_t lds = (size_t)a.W * sizeof(double2);
+    allow_lds(k_comb, lds);
+    hipLaunchKernelGGL(k_comb, dim3(a.S * a.comb_loops), dim3(kBlock), lds, st, a);
+    if (launched("comb") != 0) return -1;
+  } else {
+    const int64_t total = (int64_t)a.S * a.comb_loops * a.W;
+    hipLaunchKernelGGL(k_comb_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       a);
+    if (launched("comb_gather") != 0) return -1;
+    if (radix2_fft64(a.comb_buf, a.comb_tmp, a.W, (int64_t)a.S * a.comb_loops, 1, a.W, a.W,
+                     a.tw64, a.tw_nmax, st) != 0) {
+      set_error("comb FFT launch failed");
+      return -1;
+    }
+    hipLaunchKernelGGL(k_comb_select_global, dim3(a.S * a.comb_loops), dim3(kBlock), 0, st, a);
+    if (launched("comb_select") != 0) return -1;
+  }
   if (a.comb_loops > 1) {
     const size_t lb = (size_t)((a.W + 31) / 32) * 4;
     hipLaunchKernelGGL(k_comb_merge, dim3(a.S), dim3(kBlock), lb, st, a);
@@ -881,18 +924,25 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part)
         launches++;
       }
     } else {
-      if (n64 > 0) {
-        allow_lds(k_bucket_fft<false>, (size_t)B64 * sizeof(double2));
-        hipLaunchKernelGGL(k_bucket_fft<false>, dim3(a.S * n64), dim3(kBlock),
-                           (size_t)B64 * sizeof(double2), st, a, 0, n64);
-        if (launched("bucket_fft64") != 0) return -1;
-        launches++;
+      // the location and estimation loops as two batches (their bucket
+      // counts may differ): in LDS up to the LDS size, else radix-2 runs
+      // plus global passes
+      int groups[2][3] = {{0, a.Lloc, a.B_loc}, {a.Lloc, a.L, a.B_est}};
+      if (a.B_loc == a.B_est && (n64 == 0 || n64 == a.L)) {  // one batch
+        groups[0][1] = a.L;
+        groups[1][1] = a.L;
       }
-      if (n64 < a.L) {
-        allow_lds(k_bucket_fft<true>, (size_t)B32 * sizeof(float2));
-        hipLaunchKernelGGL(k_bucket_fft<true>, dim3(a.S * (a.L - n64)), dim3(kBlock),
-                           (size_t)B32 * sizeof(float2), st, a, n64, a.L - n64);
-        if (launched("bucket_fft32") != 0) return -1;
+      for (const auto &g : groups) {
+        const int l0 = g[0], l1 = g[1], B = g[2];
+        if (l1 <= l0) continue;
+        const int64_t off = loop_offset_host(l0, a.Lloc, a.B_loc, a.B_est);
+        // a group never straddles n64: it is either Lloc, L or 0
+        const int rc = l0 < n64
+                           ? radix2_fft64(a.xs64 + off, a.tmp64 ? a.tmp64 + off : nullptr, B,
+                                          a.S, l1 - l0, a.xs, B, a.tw64, a.tw_nmax, st)
+                           : radix2_fft32(a.xs32 + off, a.tmp32 ? a.tmp32 + off : nullptr, B,
+                                          a.S, l1 - l0, a.xs, B, a.tw32, a.tw_nmax, st);
+        if (rc != 0) return launch_failed("bucket FFT");
         launches++;
       }
     }
@@ -918,9 +968,13 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part)
 
 int select_vote_v1(const V12Launch &a, hipStream_t st)
 {
-  allow_lds(k_select, (size_t)a.B_loc * sizeof(double));
-  hipLaunchKernelGGL(k_select, dim3(a.S * a.Lloc), dim3(kBlock),
-                     (size_t)a.B_loc * sizeof(double), st, a);
+  if (a.B_loc <= kMaxLdsSelect) {
+    allow_lds(k_select<false>, (size_t)a.B_loc * sizeof(double));
+    hipLaunchKernelGGL(k_select<false>, dim3(a.S * a.Lloc), dim3(kBlock),
+                       (size_t)a.B_loc * sizeof(double), st, a);
+  } else {
+    hipLaunchKernelGGL(k_select<true>, dim3(a.S * a.Lloc), dim3(kBlock), 0, st, a);
+  }
   if (launched("select") != 0) return -1;
   const size_t lb = (size_t)a.Lloc * ((a.B_loc + 31) / 32) * 4;
   hipLaunchKernelGGL(k_vote_v1, dim3(a.S), dim3(kBlock), lb, st, a);

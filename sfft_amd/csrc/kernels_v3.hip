@@ -405,7 +405,10 @@ __global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a)
   __syncthreads();
 
   // ---- Mansour filter (cf3 96-127): two aliased sub-samplings, W-pt FFTs
-  {
+  if (a.ms_pre) {
+    const double2 *pre = a.ms_pre + (int64_t)s * 2 * W;
+    for (int i = threadIdx.x; i < 2 * W; i += blockDim.x) c.MS[2 * (i % W) + i / W] = pre[i];
+  } else {
     const int sigma = n / W, logW = ilog2v(W);
     for (int l = 0; l < 2; l++) {
       for (int i = threadIdx.x; i < W; i += blockDim.x)
@@ -608,8 +611,33 @@ __global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a)
   }
 }
 
+// Sub-samples of both Mansour loops for the pre-pass: ms_pre[(s*2+l)*W + i]
+// = x[(init_offset + l + i n/W) mod n] (cf3 96-127).
+__global__ void k_v3_mansour_gather(V3Launch a)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)a.S * 2 * a.W) return;
+  const int i = (int)(t % a.W);
+  const int l = (int)((t / a.W) & 1);
+  const int s = (int)(t / (2 * (int64_t)a.W));
+  const unsigned g0 = (unsigned)a.params[(int64_t)s * 6 + 4];
+  const unsigned sigma = (unsigned)(a.n / a.W);
+  a.ms_pre[t] = a.x[s][(g0 + (unsigned)l + (unsigned)i * sigma) & ((unsigned)a.n - 1)];
+}
+
 int v3(const V3Launch &a, hipStream_t st)
 {
+  if (a.ms_pre) {
+    const int64_t total = (int64_t)a.S * 2 * a.W;
+    hipLaunchKernelGGL(k_v3_mansour_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       st, a);
+    if (hipGetLastError() != hipSuccess ||
+        radix2_fft64(a.ms_pre, a.ms_tmp, a.W, (int64_t)a.S * 2, 1, a.W, a.W, a.tw64, a.tw_nmax,
+                     st) != 0) {
+      set_error("v3: Mansour pre-pass launch failed");
+      return -1;
+    }
+  }
   if (a.lds_bytes > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_v3),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);

@@ -213,6 +213,23 @@ class sfft:
             raise SfftError(_lib.last_error())
         return np.array([outs[i].count for i in range(num)], dtype=np.int64)
 
+    def filter(self, which):
+        """The plan's filter `which` (0 location / Gauss-1, 1 estimation /
+        Gauss-2): (taps, full n-point response with zeros outside the
+        offsets the exec reads)."""
+        half = C.c_int(0)
+        w = self._L.sfft_plan_filter(self.sfft_plan, which, None, 0, None, 0, C.byref(half))
+        if w < 0:
+            raise SfftError(_lib.last_error())
+        t = np.zeros(w, np.complex128)
+        r = np.zeros(2 * half.value + 1, np.complex128)
+        self._L.sfft_plan_filter(self.sfft_plan, which, t.ctypes.data_as(C.c_void_p), w,
+                                 r.ctypes.data_as(C.c_void_p), len(r), C.byref(half))
+        full = np.zeros(self.length, np.complex128)
+        d = np.arange(-half.value, half.value + 1)
+        full[d % self.length] = r
+        return t, full
+
     STAGES = ["comb", "fold", "bucket_fft", "select_vote", "estimate", "v3"]
 
     def enable_timing(self, enabled=True):

@@ -31,6 +31,18 @@ CASES = [
     (1, 1 << 18, 300, 20.0),    # v1 on the default parameters (k > 50)
 ]
 
+# Transforms and selections above one workgroup's LDS: bucket FFTs as LDS
+# radix-2 runs plus global passes, comb FFTs likewise, v1 selection over an
+# HBM scratch array.
+LARGE = [
+    (1, 1 << 22, 1500, 30.0),   # B = 16384: one global pass (FP64)
+    (1, 1 << 24, 2000, 30.0),   # B = 32768: two passes, selection in HBM
+    (1, 1 << 24, 50, None),     # B_loc = 16384, B_est = 1024
+    (2, 1 << 24, 10, 30.0),     # W_comb = 2^18: comb FFT with five passes
+    (2, 1 << 24, 1000, 30.0),   # v2 with B = 16384 (the comb prefill
+                                # locates 2000 * n/W = every frequency)
+]
+
 
 def _gpu_plan(version, n, k, precision="fp64"):
     return sfft_amd.sfft(n, k, version, strict=False, precision=precision,
@@ -46,10 +58,10 @@ def _oracle_run(version, n, k, x, filters=None):
     return P.exec(x, a, ai, off, trace=True)
 
 
-def _lib_filters(n, k, version):
-    p = oracle.v12_params(n, k, version == 2)
-    lt, lf = sfft_amd.design_filter(p.lobefrac_loc, p.tol_loc, n, p.b_loc)
-    et, ef = sfft_amd.design_filter(p.lobefrac_est, p.tol_est, n, p.b_est)
+def _plan_filters(plan):
+    """The taps and responses the plan itself uses (for the oracle)."""
+    lt, lf = plan.filter(0)
+    et, ef = plan.filter(1)
     return lt, lf, et, ef
 
 
@@ -67,12 +79,12 @@ def _run_gpu_dense(torch, plan, x):
     return od.cpu().numpy()
 
 
-@pytest.mark.parametrize("version,n,k,snr", CASES)
+@pytest.mark.parametrize("version,n,k,snr", CASES + LARGE)
 def test_fp64_bit_exact_given_filters(torch_cuda, version, n, k, snr):
     print(f"case v{version} n={n} k={k} snr={snr}", flush=True)
     x, f, v = signal_np(n, k, seed=1000 + n % 97 + k, snr_db=snr)
-    ref = _oracle_run(version, n, k, x, _lib_filters(n, k, version))
     plan = _gpu_plan(version, n, k)
+    ref = _oracle_run(version, n, k, x, _plan_filters(plan))
     out = _run_gpu_dense(torch_cuda, plan, x)
     loc_ref = _located(ref, version)
     loc_gpu = np.nonzero(out)[0]
@@ -81,13 +93,13 @@ def test_fp64_bit_exact_given_filters(torch_cuda, version, n, k, snr):
     assert diff == 0.0, diff
 
 
-@pytest.mark.parametrize("version,n,k,snr", CASES)
+@pytest.mark.parametrize("version,n,k,snr", CASES + LARGE[:4])
 def test_fp64_buckets_bit_exact_given_filters(torch_cuda, version, n, k, snr):
     """Fold + bucket FFT output of every loop equals the oracle's x_samp."""
     print(f"case v{version} n={n} k={k} snr={snr}", flush=True)
     x, f, v = signal_np(n, k, seed=4000 + k, snr_db=snr)
-    ref = _oracle_run(version, n, k, x, _lib_filters(n, k, version))
     plan = _gpu_plan(version, n, k)
+    ref = _oracle_run(version, n, k, x, _plan_filters(plan))
     plan.execute_sparse_host(x)
     got = plan.debug_buckets(0)
     assert len(got) == len(ref["x_samp"])
@@ -119,7 +131,8 @@ def test_fp64_parity_independent_oracle(torch_cuda, version, n, k, snr):
         assert err < 1e-6, err
 
 
-@pytest.mark.parametrize("version,n,k,snr", [c for c in CASES if c[1] >= 1 << 16])
+@pytest.mark.parametrize("version,n,k,snr",
+                         [c for c in CASES if c[1] >= 1 << 16] + LARGE[:4])
 def test_fp32_refine_matches_fp64(torch_cuda, version, n, k, snr):
     print(f"case v{version} n={n} k={k} snr={snr}", flush=True)
     x, f, v = signal_np(n, k, seed=3000 + k, snr_db=snr)
@@ -160,5 +173,5 @@ def test_host_pointers_dense(torch_cuda):
     x, f, v = signal_np(n, k, seed=5)
     plan = sfft_amd.sfft(n, k, 1, seed=(17, 4242))
     out = plan.execute(x)
-    ref = _oracle_run(1, n, k, x, _lib_filters(n, k, 1))
+    ref = _oracle_run(1, n, k, x, _plan_filters(plan))
     assert np.array_equal(out, ref["out"])

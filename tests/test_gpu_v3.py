@@ -19,13 +19,15 @@ CASES = [
     (1 << 20, 200),
     (1 << 22, 500),
     (1 << 24, 200),     # one signal of the n=2^24 v3 workload
+    (1 << 24, 1000),    # the batched v3 workload's largest k
+    (1 << 20, 2000),    # W_Man = 16384: Mansour FFTs before the kernel
 ]
 
 
-def _lib_filters_v3(n, k):
-    p = oracle.v3_params(n, k)
-    t1, f1 = sfft_amd.design_filter(p.lobefrac_g1, p.tol_g1, n, p.b_g1)
-    t2, f2 = sfft_amd.design_filter(p.lobefrac_g2, p.tol_g2, n, p.b_g2)
+def _plan_filters(plan):
+    """The taps and responses the plan itself uses (for the oracle)."""
+    t1, f1 = plan.filter(0)
+    t2, f2 = plan.filter(1)
     return t1, f1, t2, f2
 
 
@@ -40,8 +42,8 @@ def _oracle_v3(n, k, x, filters):
 def test_v3_parity(torch_cuda, n, k):
     print(f"case v3 n={n} k={k}", flush=True)
     x, f, v = signal_np(n, k, seed=5000 + k)
-    ref = _oracle_v3(n, k, x, _lib_filters_v3(n, k))
     plan = sfft_amd.sfft(n, k, 3, seed=(17, 4242))
+    ref = _oracle_v3(n, k, x, _plan_filters(plan))
     fr, va = plan.execute_sparse_host(x)
     if ref["early"]:
         assert len(fr) == 0
@@ -79,7 +81,7 @@ def test_v3_dense_host(torch_cuda):
     x, f, v = signal_np(n, k, seed=77)
     plan = sfft_amd.sfft(n, k, 3, seed=(17, 4242))
     out = plan.execute(x)
-    ref = _oracle_v3(n, k, x, _lib_filters_v3(n, k))
+    ref = _oracle_v3(n, k, x, _plan_filters(plan))
     if ref["early"]:
         assert np.count_nonzero(out) == 0
     else:
