@@ -20,6 +20,13 @@ EXPORTS = [
     "sfft_plan_enable_timing", "sfft_plan_stage_times", "sfft_plan_filter",
 ]
 
+# Every symbol include/sfft_server.h declares.
+SERVER_EXPORTS = [
+    "sfft_output_capacity", "sfft_server_create", "sfft_server_prepare",
+    "sfft_server_submit", "sfft_server_poll", "sfft_server_wait",
+    "sfft_server_get_stats", "sfft_server_destroy",
+]
+
 SFFT_VERSION_1, SFFT_VERSION_2, SFFT_VERSION_3 = 0, 1, 2
 SFFT_PRECISION_FP64, SFFT_PRECISION_FP32_REFINE = 0, 1
 
@@ -27,6 +34,21 @@ SFFT_PRECISION_FP64, SFFT_PRECISION_FP32_REFINE = 0, 1
 class SfftPlan(C.Structure):
     _fields_ = [("version", C.c_int), ("n", C.c_uint), ("k", C.c_uint),
                 ("data", C.c_void_p)]
+
+
+class SfftRequest(C.Structure):
+    _fields_ = [("n", C.c_int), ("k", C.c_int), ("version", C.c_int),
+                ("precision", C.c_int), ("inp", C.c_void_p), ("freq", C.c_void_p),
+                ("val", C.c_void_p), ("capacity", C.c_int64),
+                ("random_seed", C.c_uint), ("drand48_seed", C.c_long)]
+
+
+class SfftServerStats(C.Structure):
+    _fields_ = [(name, C.c_int64) for name in (
+        "submitted", "completed", "rejected", "failed", "batches",
+        "batched_requests", "plans_created", "plans_evicted", "plans_resident",
+        "hbm_budget", "hbm_in_use", "hbm_peak", "max_concurrent_batches")] + [
+        ("plan_build_ms", C.c_double)]
 
 
 class SfftOutput(C.Structure):
@@ -76,6 +98,18 @@ def load():
     L.sfft_plan_enable_timing.argtypes = [PP, C.c_int]
     L.sfft_plan_filter.argtypes = [PP, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_int)]
     L.sfft_plan_stage_times.argtypes = [PP, P, P]
+    L.sfft_output_capacity.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.sfft_output_capacity.restype = C.c_int64
+    L.sfft_server_create.argtypes = [C.c_int64, C.c_int, C.c_int]
+    L.sfft_server_create.restype = P
+    L.sfft_server_prepare.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.sfft_server_submit.argtypes = [P, C.POINTER(SfftRequest)]
+    L.sfft_server_submit.restype = C.c_int64
+    L.sfft_server_poll.argtypes = [P, C.c_int64]
+    L.sfft_server_wait.argtypes = [P, C.c_int64, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double)]
+    L.sfft_server_get_stats.argtypes = [P, C.POINTER(SfftServerStats)]
+    L.sfft_server_destroy.argtypes = [P]
     _lib = L
     return L
 

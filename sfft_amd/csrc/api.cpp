@@ -74,14 +74,14 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
     // inputs
     int staged = 0;
     for (int s = 0; s < S; s++) staged += !in_dev[c0 + s];
-    if (staged && !P->ws_stage.ensure((size_t)staged * n * 16)) return -1;
+    if (staged && !P->ws->stage.ensure((size_t)staged * n * 16)) return -1;
     int slot = 0;
     for (int s = 0; s < S; s++) {
       const complex_t *src = in[c0 + s];
       if (in_dev[c0 + s]) {
         ins[s] = reinterpret_cast<const double2 *>(src);
       } else {
-        double2 *dst = P->ws_stage.as<double2>() + (size_t)slot++ * n;
+        double2 *dst = P->ws->stage.as<double2>() + (size_t)slot++ * n;
         if (hipMemcpyAsync(dst, src, n * 16, hipMemcpyHostToDevice, st) != hipSuccess) {
           set_error("input copy failed");
           return -1;
@@ -118,14 +118,14 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
       }
     }
     if (nws) {
-      if (!P->ws_out_freq.ensure((size_t)nws * cap * 4) ||
-          !P->ws_out_val.ensure((size_t)nws * cap * 16))
+      if (!P->ws->out_freq.ensure((size_t)nws * cap * 4) ||
+          !P->ws->out_val.ensure((size_t)nws * cap * 16))
         return -1;
       int q = 0;
       for (int s = 0; s < S; s++) {
         if (!via_ws[s]) continue;
-        fr[s] = P->ws_out_freq.as<int32_t>() + (size_t)q * cap;
-        va[s] = P->ws_out_val.as<double2>() + (size_t)q * cap;
+        fr[s] = P->ws->out_freq.as<int32_t>() + (size_t)q * cap;
+        va[s] = P->ws->out_val.as<double2>() + (size_t)q * cap;
         q++;
       }
     }
@@ -400,10 +400,10 @@ int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
   std::vector<double2> t64(std::min(m, n64));
   std::vector<float2> t32(std::max<int64_t>(0, m - n64));
   if ((!t64.empty() &&
-       hipMemcpy(t64.data(), P->ws_xsamp.as<double2>() + (size_t)signal * xs,
+       hipMemcpy(t64.data(), P->ws->xsamp.as<double2>() + (size_t)signal * xs,
                  t64.size() * 16, hipMemcpyDeviceToHost) != hipSuccess) ||
       (!t32.empty() &&
-       hipMemcpy(t32.data(), P->ws_xsamp32.as<float2>() + (size_t)signal * xs + n64,
+       hipMemcpy(t32.data(), P->ws->xsamp32.as<float2>() + (size_t)signal * xs + n64,
                  t32.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)) {
     set_error("bucket copy failed");
     return -1;

@@ -141,14 +141,30 @@ static std::vector<double2> est_window(const std::vector<cd> &freq, int n, int B
 }
 
 // ---------------------------------------------------------------- plan
-int64_t Plan::output_capacity() const
+static int pow2_ceil(int64_t v)
 {
-  if (version == SFFT_VERSION_3) return v3_Mcap;
-  const V12Params &p = v12;
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+int64_t v12_output_capacity(const V12Params &p)
+{
   if (p.comb) return (int64_t)p.comb_loops * p.B_thresh * (p.n / p.W_comb);
   // v1: every hit is selected in >= thr of the location loops.
   const int64_t sel = (int64_t)p.loc_loops * p.B_thresh * (p.n / p.B_loc);
   return std::min<int64_t>(p.n, sel / std::max(1, p.thr_loops) + 1);
+}
+
+// v3 map of located frequencies: room for several passes' spurious finds.
+int64_t v3_map_capacity(const V3Params &q)
+{
+  return pow2_ceil(8LL * q.k + q.W_Man + 2LL * q.B_g1 + 2LL * q.B_g2 + 64);
+}
+
+int64_t Plan::output_capacity() const
+{
+  return version == SFFT_VERSION_3 ? v3_Mcap : v12_output_capacity(v12);
 }
 
 int64_t Plan::plan_bytes() const
@@ -215,13 +231,6 @@ static std::vector<cd> twiddles(int nmax)
   return tw;
 }
 
-static int pow2_ceil(int64_t v)
-{
-  int p = 1;
-  while (p < v) p <<= 1;
-  return p;
-}
-
 // v3 plan tables (src/sfft.cc:513-560): two filters with b = n/B.
 static int make_v3_tables(Plan *P)
 {
@@ -236,8 +245,7 @@ static int make_v3_tables(Plan *P)
   keep_filter(P, 0, f1, n, P->v3_half1);
   keep_filter(P, 1, f2, n, P->v3_half2);
   P->tw_nmax = std::max(q.W_Man, std::max(q.B_g1, q.B_g2));
-  // map of located frequencies: room for several passes' spurious finds
-  P->v3_Mcap = pow2_ceil(8LL * q.k + q.W_Man + 2LL * q.B_g1 + 2LL * q.B_g2 + 64);
+  P->v3_Mcap = (int)v3_map_capacity(q);
   P->v3_Hcap = 2 * P->v3_Mcap;
   P->v3_Fcap = std::max(q.W_Man, std::max(q.B_g1, q.B_g2));
   P->v3_Rcap = std::max(P->v3_Mcap, P->v3_Fcap);
@@ -328,22 +336,80 @@ Plan *make_plan(int n, int k, sfft_version v)
   return P;
 }
 
+size_t Workspace::device_bytes() const
+{
+  const DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
+                          &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
+                          &comb, &mag, &tmp64, &tmp32};
+  size_t b = 0;
+  for (const DevBuf *d : bufs) b += d->bytes;
+  return b;
+}
+
+void Workspace::release()
+{
+  DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
+                    &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
+                    &comb, &mag, &tmp64, &tmp32};
+  for (DevBuf *d : bufs) d->release();
+  if (pinned) (void)hipHostFree(pinned);
+  pinned = nullptr;
+  pinned_bytes = 0;
+}
+
+Plan *clone_plan(const Plan *src, Workspace *ws, hipStream_t stream)
+{
+  Plan *P = new Plan;
+  P->version = src->version;
+  P->n = src->n;
+  P->k = src->k;
+  P->v12 = src->v12;
+  P->v3 = src->v3;
+  P->precision = src->precision;
+  P->fold_mode = src->fold_mode;
+  P->stream_blocks = src->stream_blocks;
+  P->ref_conj = src->ref_conj;
+  P->stream = stream;
+  P->own_stream = false;
+  P->owns_tables = false;
+  P->taps_loc = src->taps_loc;
+  P->taps_est = src->taps_est;
+  P->taps_loc32 = src->taps_loc32;
+  P->taps_est32 = src->taps_est32;
+  P->win_loc = src->win_loc;
+  P->win_est = src->win_est;
+  P->tw64 = src->tw64;
+  P->tw32 = src->tw32;
+  P->tw_nmax = src->tw_nmax;
+  P->v3_time1 = src->v3_time1;
+  P->v3_time2 = src->v3_time2;
+  P->v3_freq1 = src->v3_freq1;
+  P->v3_freq2 = src->v3_freq2;
+  P->v3_half1 = src->v3_half1;
+  P->v3_half2 = src->v3_half2;
+  P->v3_Mcap = src->v3_Mcap;
+  P->v3_Hcap = src->v3_Hcap;
+  P->v3_Fcap = src->v3_Fcap;
+  P->v3_Rcap = src->v3_Rcap;
+  P->v3_lds = src->v3_lds;
+  P->ws = ws;
+  return P;
+}
+
 void free_plan(Plan *P)
 {
   if (!P) return;
-  if (P->stream) (void)hipStreamSynchronize(P->stream);
-  DevBuf *bufs[] = {&P->taps_loc, &P->taps_est, &P->taps_loc32, &P->taps_est32,
-                    &P->win_loc, &P->win_est, &P->tw64, &P->tw32,
-                    &P->v3_time1, &P->v3_time2, &P->v3_freq1, &P->v3_freq2,
-                    &P->ws_params, &P->ws_xsamp, &P->ws_xsamp32, &P->ws_approved,
-                    &P->ws_numcomb, &P->ws_J, &P->ws_hits,
-                    &P->ws_ptrs, &P->ws_out_freq, &P->ws_out_val,
-                    &P->ws_comb, &P->ws_mag, &P->ws_tmp64, &P->ws_tmp32,
-                    &P->ws_count, &P->ws_stage, &P->ws_v3};
-  for (DevBuf *b : bufs) b->release();
+  // a clone's stream belongs to its server worker, which waits for it
+  if (P->stream && (P->own_stream || P->owns_tables)) (void)hipStreamSynchronize(P->stream);
+  if (P->owns_tables) {
+    DevBuf *bufs[] = {&P->taps_loc, &P->taps_est, &P->taps_loc32, &P->taps_est32,
+                      &P->win_loc, &P->win_est, &P->tw64, &P->tw32,
+                      &P->v3_time1, &P->v3_time2, &P->v3_freq1, &P->v3_freq2};
+    for (DevBuf *b : bufs) b->release();
+  }
+  P->own_ws.release();
   if (P->ev_made)
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
-  if (P->pinned) (void)hipHostFree(P->pinned);
   if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
 }
@@ -351,12 +417,12 @@ void free_plan(Plan *P)
 // ---------------------------------------------------------------- exec
 static int ensure_pinned(Plan *P, size_t bytes)
 {
-  if (bytes <= P->pinned_bytes) return 0;
-  if (P->pinned) (void)hipHostFree(P->pinned);
-  P->pinned = nullptr;
-  P->pinned_bytes = 0;
-  HIP_OK(hipHostMalloc(&P->pinned, bytes, hipHostMallocDefault));
-  P->pinned_bytes = bytes;
+  if (bytes <= P->ws->pinned_bytes) return 0;
+  if (P->ws->pinned) (void)hipHostFree(P->ws->pinned);
+  P->ws->pinned = nullptr;
+  P->ws->pinned_bytes = 0;
+  HIP_OK(hipHostMalloc(&P->ws->pinned, bytes, hipHostMallocDefault));
+  P->ws->pinned_bytes = bytes;
   return 0;
 }
 
@@ -370,7 +436,7 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   const size_t tab_bytes = ((size_t)S * sizeof(void *) + 255) & ~size_t(255);
   const size_t cnt_bytes = ((size_t)S * sizeof(int64_t) + 255) & ~size_t(255);
   if (ensure_pinned(P, param_bytes + 4 * tab_bytes + cnt_bytes + (size_t)S * 4) != 0) return -1;
-  char *pin = reinterpret_cast<char *>(P->pinned);
+  char *pin = reinterpret_cast<char *>(P->ws->pinned);
   int *hp = reinterpret_cast<int *>(pin);
   const void **t_in = reinterpret_cast<const void **>(pin + param_bytes);
   const void **t_freq = reinterpret_cast<const void **>(pin + param_bytes + tab_bytes);
@@ -380,16 +446,17 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   int *h_status = reinterpret_cast<int *>(pin + param_bytes + 4 * tab_bytes + cnt_bytes);
   for (int s = 0; s < S; s++) {
     int *d = hp + (size_t)s * 6;
+    Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
     int a = 0;
-    while (gcd_int(a, n) != 1) a = (int)(P->rng.next_random() % n);
+    while (gcd_int(a, n) != 1) a = (int)(R.next_random() % n);
     const int ai = mod_inverse_int(a, n);
-    const int b = (int)(P->rng.next_random() % n);
+    const int b = (int)(R.next_random() % n);
     d[0] = a;
     d[1] = ai;
     d[2] = b;
     d[3] = (int)(((int64_t)ai * b) % n);  // ((ai*b) % n + n) % n with int wrap
-    d[4] = (int)(unsigned)std::floor(P->rng.next_drand48() * n);
-    d[5] = (int)(unsigned)std::floor(P->rng.next_drand48() * n);
+    d[4] = (int)(unsigned)std::floor(R.next_drand48() * n);
+    d[5] = (int)(unsigned)std::floor(R.next_drand48() * n);
     t_in[s] = io.in[s];
     t_freq[s] = io.freq ? io.freq[s] : nullptr;
     t_val[s] = io.val ? io.val[s] : nullptr;
@@ -408,17 +475,17 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   const size_t b_pre = big_ms ? al(Sz * 2 * q.W_Man * 16) : 0;
   const size_t total = b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi +
                        b_ph + 2 * b_pre;
-  if (!P->ws_params.ensure(param_bytes) || !P->ws_ptrs.ensure(4 * tab_bytes) ||
-      !P->ws_v3.ensure(total) || !P->ws_count.ensure(cnt_bytes) ||
-      !P->ws_v3_status.ensure((size_t)S * 4))
+  if (!P->ws->params.ensure(param_bytes) || !P->ws->ptrs.ensure(4 * tab_bytes) ||
+      !P->ws->v3.ensure(total) || !P->ws->count.ensure(cnt_bytes) ||
+      !P->ws->v3_status.ensure((size_t)S * 4))
     return -1;
   hipStream_t st = P->stream;
-  HIP_OK(hipMemcpyAsync(P->ws_params.p, hp, (size_t)S * 6 * sizeof(int),
+  HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * 6 * sizeof(int),
                         hipMemcpyHostToDevice, st));
-  HIP_OK(hipMemcpyAsync(P->ws_ptrs.p, pin + param_bytes, 4 * tab_bytes,
+  HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
                         hipMemcpyHostToDevice, st));
-  char *dtab = reinterpret_cast<char *>(P->ws_ptrs.p);
-  char *w = reinterpret_cast<char *>(P->ws_v3.p);
+  char *dtab = reinterpret_cast<char *>(P->ws->ptrs.p);
+  char *w = reinterpret_cast<char *>(P->ws->v3.p);
   k::V3Launch a{};
   a.n = n;
   a.S = S;
@@ -429,7 +496,7 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   a.w1 = q.w_g1;
   a.w2 = q.w_g2;
   a.x = reinterpret_cast<const double2 *const *>(dtab);
-  a.params = P->ws_params.as<const int>();
+  a.params = P->ws->params.as<const int>();
   a.taps1 = P->v3_time1.as<const double2>();
   a.taps2 = P->v3_time2.as<const double2>();
   a.win1 = P->v3_freq1.as<const double2>();
@@ -469,8 +536,8 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   a.Fcap = P->v3_Fcap;
   a.Rcap = P->v3_Rcap;
   a.max_passes = 3 * 1024;
-  a.count = P->ws_count.as<int64_t>();
-  a.status = P->ws_v3_status.as<int>();
+  a.count = P->ws->count.as<int64_t>();
+  a.status = P->ws->v3_status.as<int>();
   const bool want_sparse = io.freq != nullptr || io.val != nullptr;
   a.o_freq = want_sparse ? reinterpret_cast<int32_t *const *>(dtab + tab_bytes) : nullptr;
   a.o_val = want_sparse ? reinterpret_cast<double2 *const *>(dtab + 2 * tab_bytes) : nullptr;
@@ -523,7 +590,7 @@ int exec_batch(Plan *P, BatchIO &io)
   const size_t tab_bytes = ((size_t)S * sizeof(void *) + 255) & ~size_t(255);
   const size_t cnt_bytes = (size_t)S * sizeof(int64_t);
   if (ensure_pinned(P, param_bytes + 4 * tab_bytes + cnt_bytes) != 0) return -1;
-  char *pin = reinterpret_cast<char *>(P->pinned);
+  char *pin = reinterpret_cast<char *>(P->ws->pinned);
   int *hp = reinterpret_cast<int *>(pin);
   const void **t_in = reinterpret_cast<const void **>(pin + param_bytes);
   const void **t_freq = reinterpret_cast<const void **>(pin + param_bytes + tab_bytes);
@@ -535,14 +602,15 @@ int exec_batch(Plan *P, BatchIO &io)
   const int sigma = p.comb ? n / p.W_comb : 1;
   for (int s = 0; s < S; s++) {
     int *row = hp + (size_t)s * stride;
+    Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
     for (int l = 0; l < L; l++) {
       int a = 0;
-      while (gcd_int(a, n) != 1) a = (int)(P->rng.next_random() % n);
+      while (gcd_int(a, n) != 1) a = (int)(R.next_random() % n);
       row[l] = a;
       row[L + l] = mod_inverse_int(a, n);
     }
     for (int c = 0; c < cl; c++)
-      row[2 * L + c] = (int)(unsigned)std::floor(P->rng.next_drand48() * sigma);
+      row[2 * L + c] = (int)(unsigned)std::floor(R.next_drand48() * sigma);
   }
   // v1 keeps its location loops in FP64 under fp32 (selection-critical)
   const bool loc64 = fp32 && !p.comb;
@@ -556,14 +624,14 @@ int exec_batch(Plan *P, BatchIO &io)
     t_dense[s] = io.dense ? io.dense[s] : nullptr;
   }
 
-  if (!P->ws_params.ensure(param_bytes) || !P->ws_ptrs.ensure(4 * tab_bytes) ||
-      (need64 && !P->ws_xsamp.ensure((size_t)S * xs * 16)) ||
-      (fp32 && !P->ws_xsamp32.ensure((size_t)S * xs * 8)) ||
-      !P->ws_approved.ensure((size_t)S * std::max(cl, 1) * num * 4) ||
-      !P->ws_numcomb.ensure((size_t)S * 4) ||
-      !P->ws_J.ensure((size_t)S * p.loc_loops * num * 4) ||
-      !P->ws_hits.ensure((size_t)S * (p.comb ? 1 : cap) * 4) ||
-      !P->ws_count.ensure((size_t)S * 8))
+  if (!P->ws->params.ensure(param_bytes) || !P->ws->ptrs.ensure(4 * tab_bytes) ||
+      (need64 && !P->ws->xsamp.ensure((size_t)S * xs * 16)) ||
+      (fp32 && !P->ws->xsamp32.ensure((size_t)S * xs * 8)) ||
+      !P->ws->approved.ensure((size_t)S * std::max(cl, 1) * num * 4) ||
+      !P->ws->numcomb.ensure((size_t)S * 4) ||
+      !P->ws->J.ensure((size_t)S * p.loc_loops * num * 4) ||
+      !P->ws->hits.ensure((size_t)S * (p.comb ? 1 : cap) * 4) ||
+      !P->ws->count.ensure((size_t)S * 8))
     return -1;
   // scratch for the sizes above the LDS limits
   const bool big_comb = p.comb && p.W_comb > k::kMaxLdsComb;
@@ -573,17 +641,17 @@ int exec_batch(Plan *P, BatchIO &io)
   const size_t comb_elems = big_comb ? (size_t)S * cl * p.W_comb : 0;
   const size_t mag_elems =
       std::max(comb_elems, big_sel ? (size_t)S * p.loc_loops * p.B_loc : (size_t)0);
-  if ((big_comb && !P->ws_comb.ensure(comb_elems * 2 * sizeof(double2))) ||
-      (mag_elems && !P->ws_mag.ensure(mag_elems * sizeof(double))) ||
-      (big64 && !P->ws_tmp64.ensure((size_t)S * xs * 16)) ||
-      (big32 && !P->ws_tmp32.ensure((size_t)S * xs * 8)))
+  if ((big_comb && !P->ws->comb.ensure(comb_elems * 2 * sizeof(double2))) ||
+      (mag_elems && !P->ws->mag.ensure(mag_elems * sizeof(double))) ||
+      (big64 && !P->ws->tmp64.ensure((size_t)S * xs * 16)) ||
+      (big32 && !P->ws->tmp32.ensure((size_t)S * xs * 8)))
     return -1;
   hipStream_t st = P->stream;
-  HIP_OK(hipMemcpyAsync(P->ws_params.p, hp, (size_t)S * stride * sizeof(int),
+  HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * stride * sizeof(int),
                         hipMemcpyHostToDevice, st));
-  HIP_OK(hipMemcpyAsync(P->ws_ptrs.p, pin + param_bytes, 4 * tab_bytes,
+  HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
                         hipMemcpyHostToDevice, st));
-  char *dtab = reinterpret_cast<char *>(P->ws_ptrs.p);
+  char *dtab = reinterpret_cast<char *>(P->ws->ptrs.p);
 
   k::V12Launch a{};
   a.n = n;
@@ -599,7 +667,7 @@ int exec_batch(Plan *P, BatchIO &io)
   a.comb_loops = cl;
   a.thr = p.thr_loops;
   a.x = reinterpret_cast<const double2 *const *>(dtab);
-  a.params = P->ws_params.as<const int>();
+  a.params = P->ws->params.as<const int>();
   a.taps_loc = P->taps_loc.as<const double2>();
   a.taps_est = P->taps_est.as<const double2>();
   a.taps_loc32 = P->taps_loc32.as<const float2>();
@@ -612,25 +680,25 @@ int exec_batch(Plan *P, BatchIO &io)
   a.fp32 = fp32;
   a.loc64 = loc64;
   a.ref_conj = P->ref_conj;
-  a.xs64 = need64 ? P->ws_xsamp.as<double2>() : nullptr;
-  a.xs32 = fp32 ? P->ws_xsamp32.as<float2>() : nullptr;
+  a.xs64 = need64 ? P->ws->xsamp.as<double2>() : nullptr;
+  a.xs32 = fp32 ? P->ws->xsamp32.as<float2>() : nullptr;
   a.xs = xs;
-  a.approved = P->ws_approved.as<int>();
-  a.num_comb = P->ws_numcomb.as<int>();
-  a.J = P->ws_J.as<int>();
-  a.hits = P->ws_hits.as<int>();
+  a.approved = P->ws->approved.as<int>();
+  a.num_comb = P->ws->numcomb.as<int>();
+  a.J = P->ws->J.as<int>();
+  a.hits = P->ws->hits.as<int>();
   a.hits_cap = cap;
-  a.count = P->ws_count.as<int64_t>();
+  a.count = P->ws->count.as<int64_t>();
   a.o_freq = want_sparse ? reinterpret_cast<int32_t *const *>(dtab + tab_bytes) : nullptr;
   a.o_val = want_sparse ? reinterpret_cast<double2 *const *>(dtab + 2 * tab_bytes) : nullptr;
   a.o_dense = want_dense ? reinterpret_cast<double2 *const *>(dtab + 3 * tab_bytes) : nullptr;
   a.o_cap = io.cap;
   a.est_cap = cap;
-  a.comb_buf = big_comb ? P->ws_comb.as<double2>() : nullptr;
-  a.comb_tmp = big_comb ? P->ws_comb.as<double2>() + comb_elems : nullptr;
-  a.mag_buf = mag_elems ? P->ws_mag.as<double>() : nullptr;
-  a.tmp64 = big64 ? P->ws_tmp64.as<double2>() : nullptr;
-  a.tmp32 = big32 ? P->ws_tmp32.as<float2>() : nullptr;
+  a.comb_buf = big_comb ? P->ws->comb.as<double2>() : nullptr;
+  a.comb_tmp = big_comb ? P->ws->comb.as<double2>() + comb_elems : nullptr;
+  a.mag_buf = mag_elems ? P->ws->mag.as<double>() : nullptr;
+  a.tmp64 = big64 ? P->ws->tmp64.as<double2>() : nullptr;
+  a.tmp32 = big32 ? P->ws->tmp32.as<float2>() : nullptr;
   // the fused fold + FFT keeps a whole bucket array in LDS
   a.fold_mode = std::max(p.B_loc, p.B_est) > kMaxLdsFft64 ? 1 : P->fold_mode;
   a.stream_blocks = P->stream_blocks;

@@ -94,6 +94,20 @@ struct DevBuf {
   template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
+// Per-exec device workspace (grow-only) and pinned staging.  A plan owns
+// one; plan clones run by one server worker share that worker's.
+struct Workspace {
+  DevBuf params, xsamp, xsamp32, approved, numcomb, J, hits, ptrs, out_freq, out_val, count,
+      stage, v3, v3_status;
+  // scratch of the sizes above the LDS limits (comb FFT, v1 selection,
+  // bucket FFT passes)
+  DevBuf comb, mag, tmp64, tmp32;
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+  size_t device_bytes() const;
+  void release();
+};
+
 struct Plan {
   sfft_version version = SFFT_VERSION_1;
   int n = 0, k = 0;
@@ -126,19 +140,12 @@ struct Plan {
   int v3_half1 = 0, v3_half2 = 0;
   int v3_Mcap = 0, v3_Hcap = 0, v3_Fcap = 0, v3_Rcap = 0;
   size_t v3_lds = 0;
-  DevBuf ws_v3_status;
   std::vector<int> last_status;  // per signal of the last v3 batch
 
-  // per-exec workspace (grow-only)
-  DevBuf ws_params, ws_xsamp, ws_xsamp32, ws_approved, ws_numcomb, ws_J,
-      ws_hits, ws_ptrs, ws_out_freq, ws_out_val, ws_count, ws_stage, ws_v3;
-  // scratch of the sizes above the LDS limits (comb FFT, v1 selection,
-  // bucket FFT passes)
-  DevBuf ws_comb, ws_mag, ws_tmp64, ws_tmp32;
-  int last_S = 0;  // signals in the last batch (for diagnostics)
-  std::vector<char> host_stage;
-  void *pinned = nullptr;
-  size_t pinned_bytes = 0;
+  Workspace own_ws;
+  Workspace *ws = &own_ws;  // a clone points at its server worker's
+  bool owns_tables = true;  // false for a clone: tables belong to the source
+  int last_S = 0;           // signals in the last batch (for diagnostics)
 
   // stage timing (hipEvents on the plan's stream)
   static constexpr int kStages = 6;
@@ -153,13 +160,20 @@ struct Plan {
   int64_t bytes_per_signal() const;
 };
 
+int64_t v12_output_capacity(const V12Params &p);
+int64_t v3_map_capacity(const V3Params &q);
+
 Plan *make_plan(int n, int k, sfft_version v);
+// A plan sharing src's device tables, running on `stream` with workspace ws
+// (both owned by the caller); src must outlive it.
+Plan *clone_plan(const Plan *src, Workspace *ws, hipStream_t stream);
 void free_plan(Plan *P);
 
 // One batch of signals.  Every array here is a host array of S device
 // pointers; a null freq/val/dense array (or entry) means "not wanted".
 struct BatchIO {
   int S = 0;
+  Rng *const *rng = nullptr;        // per-signal RNG streams (null: the plan's)
   const double2 *const *in = nullptr;
   int32_t *const *freq = nullptr;   // sparse frequencies, cap entries each
   double2 *const *val = nullptr;    // sparse values, cap entries each

@@ -15,8 +15,8 @@ from sfft_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _header_functions():
-    txt = open(os.path.join(ROOT, "include", "sfft.h")).read()
+def _header_functions(name="sfft.h"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(sfft_[a-z0-9_]+)\s*\(", txt)))
 
@@ -28,6 +28,27 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert sorted(_lib.EXPORTS) == declared
+
+
+def test_library_exports_every_server_symbol():
+    L = _lib.load()
+    declared = _header_functions("sfft_server.h")
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(_lib.SERVER_EXPORTS) == declared
+
+
+def test_output_capacity_without_a_plan():
+    """Feasibility and located-set capacity from the parameters alone."""
+    L = _lib.load()
+    # v2: comb_loops * 2k * n/W_Comb (n=2^22, k=64: W_Comb = 4096)
+    assert L.sfft_output_capacity(1 << 22, 64, 1) == 128 * 1024
+    # v2 with W_Comb < 2k+1: the reference's comb selection cannot run
+    assert L.sfft_output_capacity(1 << 22, 1000, 1) == -1
+    assert "W_Comb" in _lib.last_error()
+    assert L.sfft_output_capacity(1000, 10, 0) == -1
+    # v3: the map holds 8k + W_Man + 2 B_g1 + 2 B_g2 + 64, rounded up to 2^p
+    assert L.sfft_output_capacity(1 << 20, 100, 2) == 2048
 
 
 def test_enum_values_match_reference():

@@ -1,0 +1,103 @@
+"""GPU tests of the serving loop (include/sfft_server.h).
+
+Each request carries its own RNG seeds, so its result must be the same
+whether it runs alone (a plan of its own, seeded the same way) or batched
+with other requests while other batches share the device: the stated
+tolerance is exact equality of the located set and of every value, in both
+precisions (no arithmetic depends on the batch).
+"""
+import numpy as np
+import pytest
+
+import sfft_amd
+from sfft_amd.server import Server, output_capacity
+from sfft_amd.signals import signals_device
+
+pytestmark = pytest.mark.gpu
+
+# (version, n, k, snr_db, precision)
+MIX = [
+    (2, 1 << 18, 30, 30.0, "fp32_refine"),
+    (2, 1 << 20, 64, 30.0, "fp32_refine"),
+    (2, 1 << 20, 64, 30.0, "fp64"),
+    (1, 1 << 18, 100, 30.0, "fp32_refine"),
+    (1, 1 << 16, 50, None, "fp64"),
+    (3, 1 << 20, 100, None, "fp64"),
+    (3, 1 << 18, 50, None, "fp64"),
+    (2, 1 << 22, 64, 40.0, "fp32_refine"),
+]
+
+
+def _requests(torch, copies):
+    reqs = []
+    for i, (v, n, k, snr, prec) in enumerate(MIX):
+        seeds = [100 * i + c for c in range(copies)]
+        x, _, _ = signals_device(n, k, seeds, snr_db=snr)
+        for c in range(copies):
+            reqs.append((v, n, k, prec, x[c].contiguous(), (1000 + 7 * i + c, 77 + i * 3 + c)))
+    return reqs
+
+
+def _alone(torch, v, n, k, prec, x, seed):
+    plan = sfft_amd.sfft(n, k, v, strict=False, precision=prec, seed=seed)
+    fr, va = plan.make_sparse_outputs(1)
+    counts = plan.execute_many_sparse([x], fr, va)
+    torch.cuda.synchronize()
+    c = int(counts[0])
+    return fr[0, :c].cpu().numpy(), va[0, :c].cpu().numpy()
+
+
+def test_mix_equals_alone(torch_cuda):
+    torch = torch_cuda
+    reqs = _requests(torch, copies=3)
+    order = np.random.default_rng(5).permutation(len(reqs))
+    with Server(workers=3, max_batch=4) as srv:
+        tickets = {}
+        for i in order:  # interleaved keys, submitted back to back
+            v, n, k, prec, x, seed = reqs[i]
+            tickets[i] = srv.submit(x, k, v, prec, seed)
+        got = {i: srv.wait(t) for i, t in tickets.items()}
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["batched_requests"] == len(reqs)
+    assert st["batches"] < len(reqs)            # some requests shared a batch
+    assert st["max_concurrent_batches"] >= 2    # batches overlapped
+    for i, (v, n, k, prec, x, seed) in enumerate(reqs):
+        print(f"case v{v} n={n} k={k} {prec} seed={seed}", flush=True)
+        fr, va, lat = got[i]
+        f1, v1 = _alone(torch, v, n, k, prec, x, seed)
+        assert lat > 0
+        assert np.array_equal(fr.cpu().numpy(), f1)
+        assert np.array_equal(va.cpu().numpy(), v1)
+
+
+def test_small_budget_cuts_batches(torch_cuda):
+    """Under a tight HBM budget every request still completes, batches are
+    cut to fit and the accounted footprint stays within the budget."""
+    torch = torch_cuda
+    n, k = 1 << 20, 64
+    x, _, _ = signals_device(n, k, list(range(8)), snr_db=30.0)
+    budget = 48 << 20
+    with Server(hbm_budget=budget, workers=2, max_batch=8) as srv:
+        t = [srv.submit(x[i].contiguous(), k, 2, "fp32_refine", (i + 1, i + 2)) for i in range(8)]
+        res = [srv.wait(tt) for tt in t]
+        st = srv.stats()
+    assert st["completed"] == 8
+    assert st["hbm_peak"] <= budget
+    assert st["batches"] > 1
+    for i, (fr, va, lat) in enumerate(res):
+        f1, v1 = _alone(torch, 2, n, k, "fp32_refine", x[i].contiguous(), (i + 1, i + 2))
+        assert np.array_equal(fr.cpu().numpy(), f1)
+        assert np.array_equal(va.cpu().numpy(), v1)
+
+
+def test_rejects_infeasible(torch_cuda):
+    torch = torch_cuda
+    with Server(workers=1) as srv:
+        x = torch.zeros(1 << 22, dtype=torch.complex128, device="cuda")
+        with pytest.raises(sfft_amd.SfftError):
+            srv.submit(x, 1000, 2)   # v2 with W_Comb < 2k+1
+        st = srv.stats()
+    assert st["rejected"] == 1 and st["submitted"] == 0
+    with pytest.raises(sfft_amd.SfftError):
+        output_capacity(1 << 22, 1000, 2)
