@@ -15,6 +15,8 @@ Reported beside the headline (FP32 bucketization + FP64 estimation, batched):
     per-coefficient L1 error against an FP64 full FFT of the signal;
   * roofline of the dominant kernel (fold + bucket FFT) from the library's
     own HIP events on its stream;
+  * the serving loop under Poisson arrivals of mixed (n, k, version)
+    requests: p50/p99 latency and throughput in both forms;
   * the CPU oracle timed on this host (OpenMP over signals) on a bounded
     sample.
 """
@@ -48,6 +50,12 @@ def parse():
     ap.add_argument("--cpu-signals", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="Poisson arrival rate of the serving run, requests/s")
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=16)
     ap.add_argument("--only-fp32", action="store_true",
                     help="time only the headline form (for profilers)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -221,10 +229,99 @@ def main():
             "located_per_signal": int(counts32[0]),
         }
 
+    # ---------------------------------------------------------- serving
+    if not args.no_serve and not args.only_fp32:
+        del fr32, va32
+        if "fr64" in locals():
+            del fr64, va64
+        torch.cuda.empty_cache()
+        result["serving"] = serving(args, torch, dev)
+
     # ---------------------------------------------------------- CPU
     if not args.no_cpu and not args.only_fp32:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result), flush=True)
+
+
+def serving(args, torch, dev):
+    """Poisson arrivals of single-signal requests: n uniform on
+    {2^18, 2^20, 2^22, 2^24}, k = floor(10 * 200^U), v2 (SNR 30 dB) or v3
+    (noiseless) 50/50; v2 draws the plan cannot run (W_Comb < 2k+1) are
+    rejected and counted.  Inputs and plans are device-resident before the
+    arrivals start; the same trace runs in the FP32+refine and the FP64
+    forms.  Latency = submit to the end of the request's batch."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from sfft_amd import SfftError
+    from sfft_amd.server import Server, output_capacity
+    from sfft_amd.signals import signals_device
+
+    rng = np.random.default_rng(2024)
+    N = args.serve_requests
+    sizes = [1 << 18, 1 << 20, 1 << 22, 1 << 24]
+    reqs = []
+    for i in range(N):
+        n = int(sizes[rng.integers(4)])
+        k = int(np.floor(10 * 200 ** rng.random()))
+        v = 2 if rng.random() < 0.5 else 3
+        reqs.append((n, k, v))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, N))
+    caps = []
+    for (n, k, v) in reqs:
+        try:
+            caps.append(output_capacity(n, k, v))
+        except SfftError:
+            caps.append(None)
+    dummy = {}
+    xs, outs = [], []
+    for i, (n, k, v) in enumerate(reqs):
+        if caps[i] is None:
+            if n not in dummy:
+                dummy[n] = torch.zeros(n, dtype=torch.complex128, device=dev)
+            xs.append(dummy[n])
+            outs.append(None)
+            continue
+        x, _, _ = signals_device(n, k, [5000 + i], snr_db=30.0 if v == 2 else None, device=dev)
+        xs.append(x[0])
+        outs.append((torch.empty(caps[i], dtype=torch.int32, device=dev),
+                     torch.empty(caps[i], dtype=torch.complex128, device=dev)))
+    torch.cuda.synchronize()
+    keys = sorted({reqs[i] for i in range(N) if caps[i] is not None})
+    out = {"requests": N, "rate_per_s": args.serve_rate, "workers": args.serve_workers,
+           "max_batch": args.serve_batch, "distinct_plans": len(keys),
+           "rejected_infeasible": sum(c is None for c in caps),
+           "mix": "n uniform {2^18,2^20,2^22,2^24}, k=floor(10*200^U), v2 SNR 30 dB / v3 "
+                  "noiseless 50/50"}
+    for prec in ("fp32_refine", "fp64"):
+        with Server(workers=args.serve_workers, max_batch=args.serve_batch) as srv:
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(8) as ex:
+                list(ex.map(lambda key: srv.prepare(key[0], key[1], key[2], prec), keys))
+            prep_s = time.perf_counter() - t0
+            tickets, rejected = [], 0
+            start = time.perf_counter()
+            for i, (n, k, v) in enumerate(reqs):
+                dt = arrivals[i] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                try:
+                    tickets.append(srv.submit(xs[i], k, v, prec, (i + 1, 7 * i + 3),
+                                              outputs=outs[i]))
+                except SfftError:
+                    rejected += 1
+            lat = [srv.wait(t)[2] for t in tickets]
+            span = time.perf_counter() - start
+            st = srv.stats()
+        out[prec] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                     "mean_ms": float(np.mean(lat)), "completed": st["completed"],
+                     "rejected": rejected, "failed": st["failed"],
+                     "transforms_per_s": st["completed"] / span,
+                     "batches": st["batches"],
+                     "mean_batch": st["batched_requests"] / max(1, st["batches"]),
+                     "max_concurrent_batches": st["max_concurrent_batches"],
+                     "hbm_peak_gb": st["hbm_peak"] / 1e9,
+                     "plan_prepare_s": prep_s}
+    return out
 
 
 def cpu_baseline(args):

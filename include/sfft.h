@@ -53,8 +53,9 @@ void sfft_free(void *p);
  * device-resident tap tables.  Returns NULL (and sets sfft_last_error) for a
  * bad version, n not a power of two, or an (n,k) the algorithm cannot run;
  * the reference aborts or runs out of bounds on the last two.
- * fftw_optimization: FFTW_ESTIMATE (64) or FFTW_MEASURE (0); accepted for
- * compatibility, no effect. */
+ * fftw_optimization: FFTW_ESTIMATE (64) uses the schedule tuned earlier in
+ * this process for the same key, if any; FFTW_MEASURE (0) runs the autotuner
+ * (sfft_plan_autotune) on v1/v2 plans. */
 sfft_plan *sfft_make_plan(int n, int k, sfft_version version,
                           int fftw_optimization);
 
@@ -103,6 +104,23 @@ int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
 
 /* Precision of later execs of this plan (default SFFT_PRECISION_FP64). */
 int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision);
+
+/* Fold schedule of a v1/v2 plan (no counterpart in the reference; it maps
+ * src/computefourier-1.0-2.0.cc:222-275 onto the GPU).  fold_mode 0: one
+ * workgroup per (signal, loop) folds into LDS and transforms there; 1: a
+ * signal-major streamed fold with stream_blocks resident workgroups (0: twice
+ * the CU count), then the bucket FFTs.  Schedules never change results.
+ * Returns 0 or -1. */
+int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks);
+int sfft_plan_get_schedule(const sfft_plan *plan, int *fold_mode, int *stream_blocks);
+
+/* Autotuner: times the candidate schedules on `batch` synthetic signals and
+ * keeps the fastest, for this plan and for later plans of the same (n, k,
+ * version, precision) in this process.  Draws nothing from the plan's RNG
+ * streams.  sfft_make_plan with FFTW_MEASURE (0) runs it with batch 64, as
+ * FFTW_MEASURE times FFTW's candidate plans (src/sfft.cc:274-295).  Returns
+ * 0 or -1. */
+int sfft_plan_autotune(sfft_plan *plan, int batch);
 
 /* RNG streams.  By default a plan draws from the process-global glibc
  * random() and drand48() streams exactly as the reference does (a caller's

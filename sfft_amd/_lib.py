@@ -18,6 +18,7 @@ EXPORTS = [
     "sfft_plan_memory", "sfft_last_error", "sfft_plan_params",
     "sfft_derive_params", "sfft_design_filter", "sfft_debug_buckets",
     "sfft_plan_enable_timing", "sfft_plan_stage_times", "sfft_plan_filter",
+    "sfft_plan_set_schedule", "sfft_plan_get_schedule", "sfft_plan_autotune",
 ]
 
 # Every symbol include/sfft_server.h declares.
@@ -98,6 +99,9 @@ def load():
     L.sfft_plan_enable_timing.argtypes = [PP, C.c_int]
     L.sfft_plan_filter.argtypes = [PP, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_int)]
     L.sfft_plan_stage_times.argtypes = [PP, P, P]
+    L.sfft_plan_set_schedule.argtypes = [PP, C.c_int, C.c_int]
+    L.sfft_plan_get_schedule.argtypes = [PP, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.sfft_plan_autotune.argtypes = [PP, C.c_int]
     L.sfft_output_capacity.argtypes = [C.c_int, C.c_int, C.c_int]
     L.sfft_output_capacity.restype = C.c_int64
     L.sfft_server_create.argtypes = [C.c_int64, C.c_int, C.c_int]

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsfft_amd.so")
 
-HOST_SRCS = ["host_params.cpp", "host_filters.cpp", "host_plan.cpp", "api.cpp", "server.cpp"]
+HOST_SRCS = ["host_params.cpp", "host_filters.cpp", "host_plan.cpp", "api.cpp", "server.cpp",
+             "tune.cpp"]
 DEVICE_SRCS = ["kernels_v12.hip", "kernels_v3.hip", "kernels_fft.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",

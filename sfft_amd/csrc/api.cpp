@@ -192,9 +192,16 @@ sfft_plan *sfft_make_plan(int n, int k, sfft_version version,
                           int fftw_optimization)
 {
   set_error("");
-  (void)fftw_optimization;
   Plan *P = make_plan(n, k, version);
   if (!P) return nullptr;
+  if (fftw_optimization == 0) {  // FFTW_MEASURE: tune now
+    if (autotune_plan(P, 64) != 0) {
+      free_plan(P);
+      return nullptr;
+    }
+  } else {
+    apply_tuned_schedule(P);
+  }
   sfft_plan *plan = (sfft_plan *)std::malloc(sizeof(sfft_plan));
   if (!plan) {
     free_plan(P);
@@ -255,6 +262,36 @@ int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
   Plan *P = get(plan);
   if (!P || !out) return -1;
   return run(P, num, in, nullptr, out);
+}
+
+int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if ((fold_mode != 0 && fold_mode != 1) || stream_blocks < 0 || stream_blocks > (1 << 20)) {
+    set_error("fold_mode must be 0 or 1 and stream_blocks in [0, 2^20]");
+    return -1;
+  }
+  P->fold_mode = fold_mode;
+  P->stream_blocks = stream_blocks;
+  return 0;
+}
+
+int sfft_plan_get_schedule(const sfft_plan *plan, int *fold_mode, int *stream_blocks)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if (fold_mode) *fold_mode = P->fold_mode;
+  if (stream_blocks) *stream_blocks = P->stream_blocks;
+  return 0;
+}
+
+int sfft_plan_autotune(sfft_plan *plan, int batch)
+{
+  set_error("");
+  Plan *P = get(plan);
+  if (!P) return -1;
+  return autotune_plan(P, batch);
 }
 
 int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision)

@@ -164,6 +164,12 @@ int64_t v12_output_capacity(const V12Params &p);
 int64_t v3_map_capacity(const V3Params &q);
 
 Plan *make_plan(int n, int k, sfft_version v);
+// Times the candidate fold schedules of a v1/v2 plan on `batch` synthetic
+// signals and keeps the fastest (also for later plans of the same key).
+// No effect on results or on the plan's RNG streams.  0 or -1.
+int autotune_plan(Plan *P, int batch);
+// Applies a schedule tuned earlier in this process for P's key, if any.
+void apply_tuned_schedule(Plan *P);
 // A plan sharing src's device tables, running on `stream` with workspace ws
 // (both owned by the caller); src must outlive it.
 Plan *clone_plan(const Plan *src, Workspace *ws, hipStream_t stream);
@@ -291,6 +297,8 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part);
 int select_vote_v1(const V12Launch &a, hipStream_t st);
 int estimate(const V12Launch &a, hipStream_t st);
 int zero_dense(double2 *out, int64_t n, hipStream_t st);
+// total values of uniform noise in [-1, 1)^2 (synthetic tuning input)
+int fill_noise(double2 *x, int64_t total, uint64_t seed, hipStream_t st);
 }  // namespace k
 
 }  // namespace sfa

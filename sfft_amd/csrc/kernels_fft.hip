@@ -232,7 +232,29 @@ int stockham(T2 *data, T2 *tmp, int N, int64_t nb_outer, int nb_inner, int64_t s
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Uniform noise in [-1, 1) per component from a splitmix64 hash of the
+// index (synthetic signals for the autotuner).
+__global__ void k_fill_noise(double2 *x, int64_t total, uint64_t seed)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  const double re = (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  const double im = (double)(z & 0x1fffffull) * 0x1.0p-21 * 2.0 - 1.0;
+  x[i] = make_double2(re, im);
+}
+
 }  // namespace
+
+int fill_noise(double2 *x, int64_t total, uint64_t seed, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_fill_noise, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, x,
+                     total, seed);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int radix2_fft64(double2 *data, double2 *tmp, int N, int64_t nb_outer, int nb_inner, int64_t so,
                  int64_t si, const double2 *tw, int tw_nmax, hipStream_t st)

@@ -136,6 +136,23 @@ class sfft:
             raise SfftError(_lib.last_error())
         self.precision = precision
 
+    def set_schedule(self, fold_mode, stream_blocks=0):
+        """Fold schedule (0 fused, 1 streamed with stream_blocks resident
+        workgroups); never changes results."""
+        if self._L.sfft_plan_set_schedule(self.sfft_plan, fold_mode, stream_blocks) != 0:
+            raise SfftError(_lib.last_error())
+
+    def schedule(self):
+        m, b = C.c_int(0), C.c_int(0)
+        self._L.sfft_plan_get_schedule(self.sfft_plan, C.byref(m), C.byref(b))
+        return m.value, b.value
+
+    def autotune(self, batch=64):
+        """Times the fold schedules on synthetic signals, keeps the fastest."""
+        if self._L.sfft_plan_autotune(self.sfft_plan, batch) != 0:
+            raise SfftError(_lib.last_error())
+        return self.schedule()
+
     def seed(self, random_seed=17, drand48_seed=4242):
         """Private RNG streams equal to srandom(random_seed)/srand48(..)."""
         self._L.sfft_plan_seed(self.sfft_plan, random_seed, drand48_seed)

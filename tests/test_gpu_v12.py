@@ -175,3 +175,33 @@ def test_host_pointers_dense(torch_cuda):
     out = plan.execute(x)
     ref = _oracle_run(1, n, k, x, _plan_filters(plan))
     assert np.array_equal(out, ref["out"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+@pytest.mark.parametrize("version,n,k,snr", [(2, 1 << 20, 64, 30.0), (1, 1 << 18, 100, 30.0)])
+def test_schedules_give_identical_results(torch_cuda, version, n, k, snr, precision):
+    """Fused and streamed folds (any resident-block count) sum every bucket
+    in the same tap order: identical bits."""
+    print(f"case v{version} n={n} k={k} {precision}", flush=True)
+    x, f, v = signal_np(n, k, seed=6000 + k, snr_db=snr)
+    res = []
+    for mode, blocks in [(0, 0), (1, 0), (1, 96), (1, 1024)]:
+        plan = _gpu_plan(version, n, k, precision)
+        plan.set_schedule(mode, blocks)
+        assert plan.schedule() == (mode, blocks)
+        res.append(plan.execute_sparse_host(x))
+    for fr, va in res[1:]:
+        assert np.array_equal(fr, res[0][0])
+        assert np.array_equal(va, res[0][1])
+
+
+def test_autotune_keeps_draws_and_results(torch_cuda):
+    n, k = 1 << 20, 64
+    x, f, v = signal_np(n, k, seed=77, snr_db=30.0)
+    tuned = _gpu_plan(2, n, k, "fp32_refine")
+    mode, blocks = tuned.autotune(batch=8)
+    assert mode in (0, 1) and blocks >= 0
+    fresh = _gpu_plan(2, n, k, "fp32_refine")
+    a = tuned.execute_sparse_host(x)
+    b = fresh.execute_sparse_host(x)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
