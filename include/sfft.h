@@ -102,6 +102,17 @@ int sfft_exec_sparse(sfft_plan *plan, const complex_t *in, sfft_output_c *out);
 int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
                           sfft_output_c *out);
 
+/* Signals of different plans in one call (no counterpart in the
+ * reference, whose sfft_exec_many shares one plan, src/sfft.cc:139-147):
+ * signal i runs plans[i]; each plan draws for its signals in their order
+ * here.  The v3 signals of all plans run as one launch (one workgroup per
+ * signal, each reading its own plan's tables), so a batch can mix n and k;
+ * v1/v2 signals run as one batch per plan.  in[i], out[i].freq and
+ * out[i].val must be device pointers, with out[i].capacity at least
+ * sfft_plan_output_capacity(plans[i]).  Returns 0 or -1. */
+int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *const *in,
+                         sfft_output_c *out);
+
 /* Precision of later execs of this plan (default SFFT_PRECISION_FP64). */
 int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision);
 

@@ -264,6 +264,76 @@ int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
   return run(P, num, in, nullptr, out);
 }
 
+int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *const *in,
+                         sfft_output_c *out)
+{
+  set_error("");
+  if (num <= 0) return 0;
+  if (!plans || !in || !out) {
+    set_error("null argument");
+    return -1;
+  }
+  // group the signals by plan, in order of first appearance
+  std::vector<Plan *> order;
+  std::vector<std::vector<int>> members;
+  for (int i = 0; i < num; i++) {
+    Plan *P = get(plans[i]);
+    if (!P) return -1;
+    if (!in[i] || !is_device(in[i]) || !out[i].freq || !out[i].val || !is_device(out[i].freq) ||
+        !is_device(out[i].val)) {
+      set_error("sfft_exec_many_plans takes device inputs and device sparse outputs");
+      return -1;
+    }
+    if (out[i].capacity < P->output_capacity()) {
+      set_error("output capacity below the plan's");
+      return -1;
+    }
+    size_t g = 0;
+    while (g < order.size() && order[g] != P) g++;
+    if (g == order.size()) {
+      order.push_back(P);
+      members.emplace_back();
+    }
+    members[g].push_back(i);
+  }
+  const size_t G = order.size();
+  std::vector<std::vector<const double2 *>> ins(G);
+  std::vector<std::vector<int32_t *>> frs(G);
+  std::vector<std::vector<double2 *>> vas(G);
+  std::vector<std::vector<int64_t>> counts(G);
+  std::vector<BatchIO> ios(G);
+  std::vector<Plan *> v3plans;
+  std::vector<BatchIO> v3ios;
+  for (size_t g = 0; g < G; g++) {
+    for (int i : members[g]) {
+      ins[g].push_back(reinterpret_cast<const double2 *>(in[i]));
+      frs[g].push_back(out[i].freq);
+      vas[g].push_back(reinterpret_cast<double2 *>(out[i].val));
+    }
+    counts[g].assign(members[g].size(), 0);
+    BatchIO &io = ios[g];
+    io.S = (int)members[g].size();
+    io.in = ins[g].data();
+    io.freq = frs[g].data();
+    io.val = vas[g].data();
+    io.cap = order[g]->output_capacity();
+    io.counts = counts[g].data();
+    if (order[g]->version == SFFT_VERSION_3) {
+      v3plans.push_back(order[g]);
+      v3ios.push_back(io);
+    } else if (exec_batch(order[g], io) != 0) {
+      return -1;
+    }
+  }
+  if (!v3plans.empty() &&
+      exec_multi_v3(v3plans.data(), v3ios.data(), (int)v3plans.size(), v3plans[0]->stream,
+                    v3plans[0]->ws) != 0)
+    return -1;
+  for (size_t g = 0; g < G; g++)
+    for (size_t j = 0; j < members[g].size(); j++) out[members[g][j]].count = counts[g][j];
+  return 0;
+}
+
 int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks)
 {
   Plan *P = get(plan);

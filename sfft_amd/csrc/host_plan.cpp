@@ -340,7 +340,7 @@ size_t Workspace::device_bytes() const
 {
   const DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
                           &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
-                          &comb, &mag, &tmp64, &tmp32};
+                          &comb, &mag, &tmp64, &tmp32, &multi};
   size_t b = 0;
   for (const DevBuf *d : bufs) b += d->bytes;
   return b;
@@ -350,7 +350,7 @@ void Workspace::release()
 {
   DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
                     &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
-                    &comb, &mag, &tmp64, &tmp32};
+                    &comb, &mag, &tmp64, &tmp32, &multi};
   for (DevBuf *d : bufs) d->release();
   if (pinned) (void)hipHostFree(pinned);
   pinned = nullptr;
@@ -426,8 +426,16 @@ static int ensure_pinned(Plan *P, size_t bytes)
   return 0;
 }
 
-// v3: draws (cf3 800-810) in signal order, one workgroup per signal.
-static int exec_batch_v3(Plan *P, BatchIO &io)
+// A v3 batch between its uploads and its results.
+struct V3Pending {
+  k::V3Launch a{};
+  int64_t *h_counts = nullptr;
+  int *h_status = nullptr;
+};
+
+// v3: draws (cf3 800-810) in signal order, the workspace layout and the
+// uploads of one batch (one workgroup per signal) on stream st.
+static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
 {
   const int S = io.S;
   const V3Params &q = P->v3;
@@ -479,14 +487,16 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
       !P->ws->v3.ensure(total) || !P->ws->count.ensure(cnt_bytes) ||
       !P->ws->v3_status.ensure((size_t)S * 4))
     return -1;
-  hipStream_t st = P->stream;
+  pd.h_counts = h_counts;
+  pd.h_status = h_status;
   HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * 6 * sizeof(int),
                         hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
                         hipMemcpyHostToDevice, st));
   char *dtab = reinterpret_cast<char *>(P->ws->ptrs.p);
   char *w = reinterpret_cast<char *>(P->ws->v3.p);
-  k::V3Launch a{};
+  k::V3Launch &a = pd.a;
+  a = k::V3Launch{};
   a.n = n;
   a.S = S;
   a.k = q.k;
@@ -545,15 +555,45 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
   a.o_cap = io.cap;
   a.lds_bytes = P->v3_lds;
   P->last_S = S;
+  return 0;
+}
+
+// Counts and status flags back to the host (enqueued on st).
+static int v3_collect(const V3Pending &pd, int S, hipStream_t st)
+{
+  HIP_OK(hipMemcpyAsync(pd.h_counts, pd.a.count, (size_t)S * 8, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(pd.h_status, pd.a.status, (size_t)S * 4, hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+// After the stream has drained.
+static int v3_finish(Plan *P, BatchIO &io, const V3Pending &pd)
+{
+  const int S = io.S;
+  P->last_status.assign(pd.h_status, pd.h_status + S);
+  if (io.counts)
+    for (int s = 0; s < S; s++) io.counts[s] = pd.h_counts[s];
+  for (int s = 0; s < S; s++)
+    if (pd.h_status[s] & 2) {
+      set_error("v3 result map overflow (more located frequencies than the plan holds)");
+      return -1;
+    }
+  return 0;
+}
+
+static int exec_batch_v3(Plan *P, BatchIO &io)
+{
+  hipStream_t st = P->stream;
+  V3Pending pd;
+  if (v3_prepare(P, io, st, pd) != 0) return -1;
   if (P->timing && !P->ev_made) {
     for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
     P->ev_made = true;
   }
   if (P->timing) (void)hipEventRecord(P->ev[0], st);
-  if (k::v3(a, st) != 0) return -1;
+  if (k::v3(pd.a, st) != 0) return -1;
   if (P->timing) (void)hipEventRecord(P->ev[1], st);
-  HIP_OK(hipMemcpyAsync(h_counts, a.count, (size_t)S * 8, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(h_status, a.status, (size_t)S * 4, hipMemcpyDeviceToHost, st));
+  if (v3_collect(pd, io.S, st) != 0) return -1;
   HIP_OK(hipStreamSynchronize(st));
   if (P->timing) {
     float ms = 0;
@@ -562,15 +602,39 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
       P->stage_launches[5] += 1;
     }
   }
-  P->last_status.assign(h_status, h_status + S);
-  if (io.counts)
-    for (int s = 0; s < S; s++) io.counts[s] = h_counts[s];
-  for (int s = 0; s < S; s++)
-    if (h_status[s] & 2) {
-      set_error("v3 result map overflow (more located frequencies than the plan holds)");
+  return v3_finish(P, io, pd);
+}
+
+int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws)
+{
+  std::vector<V3Pending> pd(G);
+  std::vector<k::V3Launch> groups(G);
+  std::vector<int2> blocks;
+  for (int g = 0; g < G; g++) {
+    if (plans[g]->version != SFFT_VERSION_3) {
+      set_error("exec_multi_v3: not a v3 plan");
       return -1;
     }
-  return 0;
+    if (v3_prepare(plans[g], ios[g], st, pd[g]) != 0) return -1;
+    groups[g] = pd[g].a;
+    for (int s = 0; s < ios[g].S; s++) blocks.push_back(make_int2(g, s));
+  }
+  if (blocks.empty()) return 0;
+  if (!ws->multi.ensure(G * sizeof(k::V3Launch) + blocks.size() * sizeof(int2) + 256))
+    return -1;
+  auto *d_groups = ws->multi.as<k::V3Launch>();
+  auto *d_blocks = reinterpret_cast<int2 *>(
+      (reinterpret_cast<uintptr_t>(d_groups + G) + 255) & ~uintptr_t(255));
+  if (k::v3_multi(groups.data(), G, d_groups, d_blocks, blocks.data(), (int)blocks.size(),
+                  st) != 0)
+    return -1;
+  for (int g = 0; g < G; g++)
+    if (v3_collect(pd[g], ios[g].S, st) != 0) return -1;
+  HIP_OK(hipStreamSynchronize(st));
+  int rc = 0;
+  for (int g = 0; g < G; g++)
+    if (v3_finish(plans[g], ios[g], pd[g]) != 0) rc = -1;
+  return rc;
 }
 
 int exec_batch(Plan *P, BatchIO &io)

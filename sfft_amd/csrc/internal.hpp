@@ -102,6 +102,7 @@ struct Workspace {
   // scratch of the sizes above the LDS limits (comb FFT, v1 selection,
   // bucket FFT passes)
   DevBuf comb, mag, tmp64, tmp32;
+  DevBuf multi;  // descriptors of a multi-plan launch
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
   size_t device_bytes() const;
@@ -191,6 +192,10 @@ struct BatchIO {
 // Draws the RNG values of S signals in signal order, runs the kernels on
 // the plan's stream and waits for them.  Returns 0 or -1 (set_error).
 int exec_batch(Plan *P, BatchIO &io);
+// v3 batches of G plans (different n, k) as one launch on stream st; the
+// descriptors live in ws (not any plan's).  Plans' own workspaces hold their
+// signals' state.  Returns 0 or -1.
+int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws);
 
 // ---------------------------------------------------------------- kernels
 namespace k {
@@ -271,6 +276,10 @@ struct V3Launch {
   size_t lds_bytes;
 };
 int v3(const V3Launch &a, hipStream_t st);
+// G groups in one launch; d_groups / d_blocks: device room for the
+// descriptors and the nblocks (group, signal) pairs.
+int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,
+             const int2 *h_blocks, int nblocks, hipStream_t st);
 
 // Batched power-of-two FFTs; transform t = (outer, inner) lives at
 // data + outer*so + inner*si.  In LDS for small forward sizes (tw: the

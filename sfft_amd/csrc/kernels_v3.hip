@@ -359,12 +359,12 @@ __device__ bool map_add_found(const Ctx &c, int F, int *sh, int *s_size)
 
 }  // namespace
 
-__global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a)
+// The whole of one signal's transform, in one workgroup.
+__device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
 {
   extern __shared__ double2 lds[];
   __shared__ int sh[16];
   __shared__ int s_size, s_flag, s_cnt[3];
-  const int s = blockIdx.x;
   Ctx c;
   c.a = &a;
   c.s = s;
@@ -611,6 +611,17 @@ __global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a)
   }
 }
 
+__global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a) { v3_signal(a, blockIdx.x); }
+
+// Signals of several plans (different n, k) in one launch: block b runs
+// signal blocks[b].y of plan group blocks[b].x.
+__global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *groups,
+                                                       const int2 *blocks)
+{
+  const int2 b = blocks[blockIdx.x];
+  v3_signal(groups[b.x], b.y);
+}
+
 // Sub-samples of both Mansour loops for the pre-pass: ms_pre[(s*2+l)*W + i]
 // = x[(init_offset + l + i n/W) mod n] (cf3 96-127).
 __global__ void k_v3_mansour_gather(V3Launch a)
@@ -625,26 +636,61 @@ __global__ void k_v3_mansour_gather(V3Launch a)
   a.ms_pre[t] = a.x[s][(g0 + (unsigned)l + (unsigned)i * sigma) & ((unsigned)a.n - 1)];
 }
 
+// Mansour pre-pass of one plan group whose W exceeds the LDS size.
+static int mansour_prepass(const V3Launch &a, hipStream_t st)
+{
+  if (!a.ms_pre) return 0;
+  const int64_t total = (int64_t)a.S * 2 * a.W;
+  hipLaunchKernelGGL(k_v3_mansour_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     st, a);
+  if (hipGetLastError() != hipSuccess ||
+      radix2_fft64(a.ms_pre, a.ms_tmp, a.W, (int64_t)a.S * 2, 1, a.W, a.W, a.tw64, a.tw_nmax,
+                   st) != 0) {
+    set_error("v3: Mansour pre-pass launch failed");
+    return -1;
+  }
+  return 0;
+}
+
+static void allow_v3_lds(const void *kernel, size_t bytes)
+{
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 int v3(const V3Launch &a, hipStream_t st)
 {
-  if (a.ms_pre) {
-    const int64_t total = (int64_t)a.S * 2 * a.W;
-    hipLaunchKernelGGL(k_v3_mansour_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       st, a);
-    if (hipGetLastError() != hipSuccess ||
-        radix2_fft64(a.ms_pre, a.ms_tmp, a.W, (int64_t)a.S * 2, 1, a.W, a.W, a.tw64, a.tw_nmax,
-                     st) != 0) {
-      set_error("v3: Mansour pre-pass launch failed");
-      return -1;
-    }
-  }
-  if (a.lds_bytes > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_v3),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds_bytes);
+  if (mansour_prepass(a, st) != 0) return -1;
+  allow_v3_lds(reinterpret_cast<const void *>(k_v3), a.lds_bytes);
   hipLaunchKernelGGL(k_v3, dim3(a.S), dim3(kV3Block), a.lds_bytes, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("v3: ") + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,
+             const int2 *h_blocks, int nblocks, hipStream_t st)
+{
+  size_t lds = 0;
+  for (int g = 0; g < G; g++) {
+    if (mansour_prepass(h_groups[g], st) != 0) return -1;
+    lds = std::max(lds, h_groups[g].lds_bytes);
+  }
+  if (hipMemcpyAsync(d_groups, h_groups, (size_t)G * sizeof(V3Launch), hipMemcpyHostToDevice,
+                     st) != hipSuccess ||
+      hipMemcpyAsync(d_blocks, h_blocks, (size_t)nblocks * sizeof(int2), hipMemcpyHostToDevice,
+                     st) != hipSuccess) {
+    set_error("v3: descriptor upload failed");
+    return -1;
+  }
+  allow_v3_lds(reinterpret_cast<const void *>(k_v3_multi), lds);
+  hipLaunchKernelGGL(k_v3_multi, dim3(nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("v3 multi: ") + hipGetErrorString(e));
     return -1;
   }
   return 0;

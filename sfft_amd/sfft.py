@@ -282,3 +282,30 @@ class sfft:
             raise SfftError(_lib.last_error())
         m = min(o.count, cap)
         return fr[:m].copy(), va[:m].copy()
+
+
+def execute_many_plans(plans, xs, outputs=None):
+    """sfft_exec_many_plans: signal i of xs (device tensors) runs plans[i];
+    the v3 signals of all plans run as one launch.  outputs: list of
+    (freq, val) device tensors (allocated at each plan's capacity when
+    None).  Returns [(freq[:count], val[:count])]."""
+    import torch
+    num = len(xs)
+    assert len(plans) == num
+    for p in plans:
+        p._bind_stream(torch)
+    if outputs is None:
+        outputs = [(torch.empty(p.capacity, dtype=torch.int32, device=x.device),
+                    torch.empty(p.capacity, dtype=torch.complex128, device=x.device))
+                   for p, x in zip(plans, xs)]
+    pl = (C.c_void_p * num)(*[C.cast(p.sfft_plan, C.c_void_p).value for p in plans])
+    ins = (C.c_void_p * num)(*[x.data_ptr() for x in xs])
+    outs = (_lib.SfftOutput * num)()
+    for i, (fr, va) in enumerate(outputs):
+        outs[i].capacity = int(fr.numel())
+        outs[i].freq = fr.data_ptr()
+        outs[i].val = va.data_ptr()
+    if _lib.load().sfft_exec_many_plans(num, pl, ins, outs) != 0:
+        raise SfftError(_lib.last_error())
+    return [(fr[:min(outs[i].count, fr.numel())], va[:min(outs[i].count, va.numel())])
+            for i, (fr, va) in enumerate(outputs)]

@@ -87,3 +87,29 @@ def test_v3_dense_host(torch_cuda):
     else:
         assert np.array_equal(np.nonzero(out)[0], np.nonzero(ref["out"])[0])
         assert np.abs(out - ref["out"]).max() <= 1e-9
+
+
+def test_v3_uneven_k_one_launch(torch_cuda):
+    """Signals of plans with different k (and n) in one launch give exactly
+    what each plan gives alone (each plan draws for its signals in order)."""
+    torch = torch_cuda
+    from sfft_amd.sfft import execute_many_plans
+    keys = [(1 << 20, 200), (1 << 20, 500), (1 << 20, 1000), (1 << 18, 100)]
+    order = [0, 1, 2, 3, 2, 1, 0, 3]   # interleaved, two signals per plan
+    xs = [torch.from_numpy(signal_np(keys[g][0], keys[g][1], seed=900 + i)[0]).cuda()
+          for i, g in enumerate(order)]
+    plans = [sfft_amd.sfft(n, k, 3, seed=(17 + g, 4242 + g)) for g, (n, k) in enumerate(keys)]
+    res = execute_many_plans([plans[g] for g in order], xs)
+    torch.cuda.synchronize()
+    for g, (n, k) in enumerate(keys):
+        alone = sfft_amd.sfft(n, k, 3, seed=(17 + g, 4242 + g))
+        idx = [i for i, gg in enumerate(order) if gg == g]
+        fr, va = alone.make_sparse_outputs(len(idx))
+        counts = alone.execute_many_sparse([xs[i] for i in idx], fr, va)
+        torch.cuda.synchronize()
+        for j, i in enumerate(idx):
+            print(f"case v3 n={n} k={k} signal {i}", flush=True)
+            c = int(counts[j])
+            assert int(res[i][0].numel()) == c
+            assert torch.equal(res[i][0], fr[j, :c])
+            assert torch.equal(res[i][1], va[j, :c])
