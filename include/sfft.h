@@ -125,6 +125,15 @@ int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision);
 int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks);
 int sfft_plan_get_schedule(const sfft_plan *plan, int *fold_mode, int *stream_blocks);
 
+/* HIP graphs (no counterpart in the reference): with enabled != 0, each
+ * v1/v2 exec replays its uploads, kernels and count read-back from a graph
+ * captured on the first batch of the same shape (signal count, precision,
+ * outputs, schedule) and workspace addresses; the host still draws the RNG
+ * values and writes the pinned staging buffers the graph copies from.  Cuts
+ * the launch overhead of small batches; results are identical.  Stage timing
+ * bypasses the graphs.  Returns 0 or -1. */
+int sfft_plan_enable_graphs(sfft_plan *plan, int enabled);
+
 /* Autotuner: times the candidate schedules on `batch` synthetic signals and
  * keeps the fastest, for this plan and for later plans of the same (n, k,
  * version, precision) in this process.  Draws nothing from the plan's RNG

@@ -356,6 +356,14 @@ int sfft_plan_get_schedule(const sfft_plan *plan, int *fold_mode, int *stream_bl
   return 0;
 }
 
+int sfft_plan_enable_graphs(sfft_plan *plan, int enabled)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  P->use_graphs = enabled != 0;
+  return 0;
+}
+
 int sfft_plan_autotune(sfft_plan *plan, int batch)
 {
   set_error("");

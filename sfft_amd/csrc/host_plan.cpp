@@ -407,6 +407,7 @@ void free_plan(Plan *P)
                       &P->v3_time1, &P->v3_time2, &P->v3_freq1, &P->v3_freq2};
     for (DevBuf *b : bufs) b->release();
   }
+  for (const auto &g : P->graphs) (void)hipGraphExecDestroy(g.exec);
   P->own_ws.release();
   if (P->ev_made)
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
@@ -637,6 +638,20 @@ int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Works
   return rc;
 }
 
+// Addresses a captured batch depends on (FNV-1a over the pointers).
+static uint64_t workspace_signature(const Workspace &w)
+{
+  const void *ptrs[] = {w.params.p, w.xsamp.p, w.xsamp32.p, w.approved.p, w.numcomb.p, w.J.p,
+                        w.hits.p, w.ptrs.p, w.count.p, w.comb.p, w.mag.p, w.tmp64.p,
+                        w.tmp32.p, w.pinned};
+  uint64_t h = 1469598103934665603ull;
+  for (const void *q : ptrs) {
+    h ^= (uint64_t)(uintptr_t)q;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
 int exec_batch(Plan *P, BatchIO &io)
 {
   if (P->version == SFFT_VERSION_3) return exec_batch_v3(P, io);
@@ -711,10 +726,6 @@ int exec_batch(Plan *P, BatchIO &io)
       (big32 && !P->ws->tmp32.ensure((size_t)S * xs * 8)))
     return -1;
   hipStream_t st = P->stream;
-  HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * stride * sizeof(int),
-                        hipMemcpyHostToDevice, st));
-  HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
-                        hipMemcpyHostToDevice, st));
   char *dtab = reinterpret_cast<char *>(P->ws->ptrs.p);
 
   k::V12Launch a{};
@@ -777,27 +788,73 @@ int exec_batch(Plan *P, BatchIO &io)
     for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
     P->ev_made = true;
   }
-  auto mark = [&](int i) {
-    if (P->timing) (void)hipEventRecord(P->ev[i], st);
-  };
   int launches[5] = {0, 0, 0, 0, 0};
-  mark(0);
-  if (p.comb && k::comb(a, st) != 0) return -1;
-  launches[0] = p.comb ? (cl > 1 ? 2 : 1) : 0;
-  mark(1);
+  // the stream work of one batch: uploads, kernels, counts back
+  auto enqueue = [&](bool timed) -> int {
+    auto mark = [&](int i) {
+      if (timed) (void)hipEventRecord(P->ev[i], st);
+    };
+    HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * stride * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
+                          hipMemcpyHostToDevice, st));
+    mark(0);
+    if (p.comb && k::comb(a, st) != 0) return -1;
+    launches[0] = p.comb ? (cl > 1 ? 2 : 1) : 0;
+    mark(1);
+    if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
+    mark(2);
+    if ((launches[2] = k::fold_fft(a, st, 1)) < 0) return -1;
+    mark(3);
+    if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
+    launches[3] = p.comb ? 0 : 2;
+    mark(4);
+    if (k::estimate(a, st) != 0) return -1;
+    launches[4] = 1;
+    mark(5);
+    HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
+    return 0;
+  };
   P->last_fp32 = fp32;
   P->last_S = S;
-  if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
-  mark(2);
-  if ((launches[2] = k::fold_fft(a, st, 1)) < 0) return -1;
-  mark(3);
-  if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
-  launches[3] = p.comb ? 0 : 2;
-  mark(4);
-  if (k::estimate(a, st) != 0) return -1;
-  launches[4] = 1;
-  mark(5);
-  HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
+  if (P->use_graphs && !P->timing) {
+    // replay the batch's stream work from a graph captured on the first
+    // batch of this shape; the workspace and staging addresses are part of
+    // the key (kernel arguments and copy nodes hold them)
+    const uint64_t key = (uint64_t)S | (uint64_t)fp32 << 32 | (uint64_t)want_sparse << 33 |
+                         (uint64_t)want_dense << 34 | (uint64_t)a.fold_mode << 35 |
+                         (uint64_t)a.stream_blocks << 36;
+    const uint64_t sig = workspace_signature(*P->ws) ^
+                         ((uint64_t)io.cap * 0x9e3779b97f4a7c15ull + (uint64_t)a.ref_conj);
+    hipGraphExec_t ex = nullptr;
+    for (const auto &g : P->graphs)
+      if (g.key == key && g.sig == sig) ex = g.exec;
+    if (!ex) {
+      HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+      const int rc = enqueue(false);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc != 0 || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc == 0) set_error(std::string("graph capture: ") + hipGetErrorString(e));
+        return -1;
+      }
+      const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        set_error(std::string("graph instantiate: ") + hipGetErrorString(ei));
+        return -1;
+      }
+      if (P->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(P->graphs.front().exec);
+        P->graphs.erase(P->graphs.begin());
+      }
+      P->graphs.push_back(Plan::GraphEntry{key, sig, ex});
+    }
+    HIP_OK(hipGraphLaunch(ex, st));
+  } else if (enqueue(P->timing) != 0) {
+    return -1;
+  }
   HIP_OK(hipStreamSynchronize(st));
   if (P->timing) {
     for (int i = 0; i < 5; i++) {

@@ -148,6 +148,15 @@ struct Plan {
   bool owns_tables = true;  // false for a clone: tables belong to the source
   int last_S = 0;           // signals in the last batch (for diagnostics)
 
+  // v1/v2 batches replayed from captured HIP graphs (per batch shape and
+  // workspace addresses; at most 8 kept)
+  bool use_graphs = false;
+  struct GraphEntry {
+    uint64_t key, sig;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+
   // stage timing (hipEvents on the plan's stream)
   static constexpr int kStages = 6;
   bool timing = false;

@@ -930,7 +930,7 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part)
       int groups[2][3] = {{0, a.Lloc, a.B_loc}, {a.Lloc, a.L, a.B_est}};
       if (a.B_loc == a.B_est && (n64 == 0 || n64 == a.L)) {  // one batch
         groups[0][1] = a.L;
-        groups[1][1] = a.L;
+        groups[1][0] = a.L;  // empty
       }
       for (const auto &g : groups) {
         const int l0 = g[0], l1 = g[1], B = g[2];

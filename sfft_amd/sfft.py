@@ -147,6 +147,10 @@ class sfft:
         self._L.sfft_plan_get_schedule(self.sfft_plan, C.byref(m), C.byref(b))
         return m.value, b.value
 
+    def enable_graphs(self, enabled=True):
+        """Replay v1/v2 execs from captured HIP graphs (identical results)."""
+        self._L.sfft_plan_enable_graphs(self.sfft_plan, 1 if enabled else 0)
+
     def autotune(self, batch=64):
         """Times the fold schedules on synthetic signals, keeps the fastest."""
         if self._L.sfft_plan_autotune(self.sfft_plan, batch) != 0:

@@ -205,3 +205,29 @@ def test_autotune_keeps_draws_and_results(torch_cuda):
     a = tuned.execute_sparse_host(x)
     b = fresh.execute_sparse_host(x)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("version,n,k,snr,precision", [
+    (2, 1 << 20, 64, 30.0, "fp32_refine"), (1, 1 << 18, 100, 30.0, "fp64")])
+def test_graph_replays_match_eager(torch_cuda, version, n, k, snr, precision):
+    """Replays of the captured batch (new signals and new draws each time,
+    eager torch work in between) equal the eager execs exactly."""
+    torch = torch_cuda
+    print(f"case graphs v{version} n={n} k={k} {precision}", flush=True)
+    g = _gpu_plan(version, n, k, precision)
+    g.enable_graphs(True)
+    e = _gpu_plan(version, n, k, precision)
+    for rep in range(4):
+        xs = [torch.from_numpy(signal_np(n, k, seed=700 + 3 * rep + i, snr_db=snr)[0]).cuda()
+              for i in range(3)]
+        fg, vg = g.make_sparse_outputs(3)
+        fe, ve = e.make_sparse_outputs(3)
+        cg = g.execute_many_sparse(xs, fg, vg)
+        _ = torch.stack(xs).abs().sum().item()   # eager work between replays
+        ce = e.execute_many_sparse(xs, fe, ve)
+        torch.cuda.synchronize()
+        assert np.array_equal(cg, ce)
+        for i in range(3):
+            c = int(ce[i])
+            assert torch.equal(fg[i, :c], fe[i, :c])
+            assert torch.equal(vg[i, :c], ve[i, :c])

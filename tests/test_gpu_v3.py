@@ -51,10 +51,14 @@ def test_v3_parity(torch_cuda, n, k):
     assert np.array_equal(fr, ref["freqs"]), (len(fr), len(ref["freqs"]))
     diff = np.abs(va - ref["vals"])
     assert diff.max() <= 1e-9 * np.abs(ref["vals"]).max(), diff.max()
-    # every tone located, with its value (the reference's v3 is not conjugated)
+    # the tones located carry their values (the reference's v3 is not
+    # conjugated); at k > B_g1 (k=1000: 512 Gauss-1 buckets) the reference's
+    # loop can stop with tones unresolved, which the oracle reproduces
     got = dict(zip(fr.tolist(), va.tolist()))
-    assert all(int(ff) in got for ff in f)
-    err = max(abs(got[int(ff)] - vv) for ff, vv in zip(f, v))
+    found = [int(ff) in got for ff in f]
+    print(f"located {sum(found)} of {k} tones", flush=True)
+    assert np.mean(found) >= (1.0 if k <= 500 else 0.9)
+    err = max(abs(got[int(ff)] - vv) for ff, vv in zip(f, v) if int(ff) in got)
     assert err < 0.2, err   # quirk: Gauss-2 phase rounding at large n
 
 
