@@ -102,6 +102,17 @@ int sfft_exec_sparse(sfft_plan *plan, const complex_t *in, sfft_output_c *out);
 int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
                           sfft_output_c *out);
 
+/* Reference-compatible synthetic input (src/simulation.cc:95-112,
+ * src/timing_many.cc:117-139): k frequencies drawn with replacement as
+ * floor(drand48() * n) with X[f] = 1, x = the unnormalised inverse DFT
+ * (e^{+2 pi i f t / n}); with std_noise != 0, complex Gaussian noise added
+ * as AWGN does (src/utils.cc:250-280).  The drand48 stream is private,
+ * seeded like srand48(drand48_seed).  Host pointers: x (n values), x_f (n
+ * values or NULL), snr (NULL or the measured signal/noise power ratio, 1e9
+ * when noiseless).  Returns 0 or -1. */
+int sfft_generate_input(int n, int k, long drand48_seed, double std_noise, complex_t *x,
+                        complex_t *x_f, double *snr);
+
 /* Signals of different plans in one call (no counterpart in the
  * reference, whose sfft_exec_many shares one plan, src/sfft.cc:139-147):
  * signal i runs plans[i]; each plan draws for its signals in their order

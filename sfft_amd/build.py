@@ -16,7 +16,7 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsfft_amd.so")
 
 HOST_SRCS = ["host_params.cpp", "host_filters.cpp", "host_plan.cpp", "api.cpp", "server.cpp",
-             "tune.cpp"]
+             "tune.cpp", "host_signal.cpp"]
 DEVICE_SRCS = ["kernels_v12.hip", "kernels_v3.hip", "kernels_fft.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",

@@ -106,6 +106,8 @@ class sfft:
         if seed is not None:
             self.seed(*seed)
         self._stream = None
+        if optimization == FFTW_MEASURE:  # tune the schedule at this precision
+            self.autotune(64)
 
     # ------------------------------------------------------------ reference
     def execute(self, a):

@@ -53,3 +53,20 @@ def signals_device(n, k, seeds, snr_db=None, device="cuda"):
                 torch.randn(n, dtype=torch.float64, device=device, generator=g),
                 torch.randn(n, dtype=torch.float64, device=device, generator=g))
     return x, fs, vs
+
+
+def reference_input(n, k, seed, std_noise=0.0):
+    """The reference harness's input (src/simulation.cc:95-112): k
+    frequencies drawn with replacement as floor(drand48() * n), X[f] = 1,
+    x = unnormalised inverse DFT, plus AWGN noise of standard deviation
+    std_noise (src/utils.cc:250-280), from a drand48 stream seeded like
+    srand48(seed).  Returns (x, X, snr)."""
+    import ctypes as C
+    from . import _lib
+    x = np.empty(n, np.complex128)
+    X = np.empty(n, np.complex128)
+    snr = C.c_double(0)
+    if _lib.load().sfft_generate_input(n, k, seed, std_noise, x.ctypes.data_as(C.c_void_p),
+                                       X.ctypes.data_as(C.c_void_p), C.byref(snr)) != 0:
+        raise ValueError(_lib.last_error())
+    return x, X, snr.value

@@ -140,3 +140,38 @@ def test_python_mirror_validation():
         sfft_amd.sfft(16384.0, 50, 1)
     with pytest.raises(ValueError):
         sfft_amd.sfft(16384, 50, 1, optimization=3)
+
+
+def test_reference_input_generator():
+    """src/simulation.cc:95-112 + src/utils.cc:250-280 from a private
+    drand48 stream: frequencies equal glibc's drand48 draws after
+    srand48(seed), x is the unnormalised inverse DFT, the noise follows the
+    Box-Muller form with the stated deviation."""
+    from sfft_amd.signals import reference_input
+    n, k, seed = 1 << 12, 20, 4242
+    x, X, snr = reference_input(n, k, seed)
+    libc = C.CDLL("libc.so.6")
+    libc.drand48.restype = C.c_double
+    libc.srand48(C.c_long(seed))
+    want = sorted({int(np.floor(libc.drand48() * n)) for _ in range(k)})
+    assert np.nonzero(X)[0].tolist() == want
+    assert np.all(X[want] == 1.0)
+    assert np.abs(x - np.fft.ifft(X) * n).max() < 1e-9
+    assert snr == 1e9
+    # noisy: the first noise draws continue the same stream
+    xn, Xn, snr_n = reference_input(n, k, seed, std_noise=0.5)
+    assert np.array_equal(Xn, X)
+    u, v = libc.drand48(), libc.drand48()
+    g0 = 0.5 * np.sqrt(-2 * np.log(u)) * np.exp(2j * np.pi * v)
+    assert abs((xn[0] - x[0]) - g0) < 1e-12
+    noise = xn - x
+    assert abs(np.mean(np.abs(noise) ** 2) / (2 * 0.25) - 1) < 0.1
+    assert abs(snr_n / (np.sum(np.abs(x) ** 2) / np.sum(np.abs(noise) ** 2)) - 1) < 1e-9
+
+
+def test_tools_usage_without_gpu(capsys):
+    from sfft_amd import tools
+    assert tools.main([]) == 1
+    assert "timing_many" in capsys.readouterr().out
+    with pytest.raises(SystemExit):
+        tools.main(["timing", "-h"])
