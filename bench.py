@@ -99,6 +99,7 @@ def main():
 
     # ---------------------------------------------------------- headline
     p32 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp32_refine", seed=(17, 4242))
+    schedule32 = p32.autotune(batch=min(S, 64))  # fold schedule only; same results
     cap = p32.capacity
     fr32, va32 = p32.make_sparse_outputs(S)
     for _ in range(args.warmup):
@@ -126,7 +127,8 @@ def main():
                    "n": n, "k": k, "batch": S, "version": 2, "snr_db": args.snr,
                    "params": dict(zip(["B_loc", "B_est", "B_thresh", "W_comb",
                                        "comb_loops", "loc_loops", "est_loops",
-                                       "thr_loops", "w_loc", "w_est"], p32.params()))},
+                                       "thr_loops", "w_loc", "w_est"], p32.params())),
+                   "fold_schedule": {"mode": schedule32[0], "resident_blocks": schedule32[1]}},
     }
 
     # ---------------------------------------------------------- roofline
@@ -140,8 +142,7 @@ def main():
     fold_ms, fold_launches = stages32["fold"]
     fold_avg_ms = fold_ms / max(1, fold_launches)
     achieved = fold_bytes / (fold_avg_ms * 1e-3) / 1e9
-    fused = os.environ.get("SFFT_FOLD_MODE", "1") == "0"
-    kname = "k_fold_fft32" if fused else "k_fold_stream<true>"
+    kname = "k_fold_fft32" if schedule32[0] == 0 else "k_fold_stream<true>"
     result["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": None, "alg_bytes_per_launch": fold_bytes,
@@ -150,18 +151,29 @@ def main():
 
     if not args.only_fp32:
         # ------------------------------------------------------ latency
-        lat = []
         one_f, one_v = fr32[:1], va32[:1]
-        for i in range(args.lat_samples):
-            torch.cuda.synchronize()
-            ts = time.perf_counter()
-            p32.execute_many_sparse([xs[i % S]], one_f, one_v)
-            lat.append((time.perf_counter() - ts) * 1e3)
+
+        def latency():
+            lat = []
+            p32.execute_many_sparse([xs[0]], one_f, one_v)  # warm (captures a graph)
+            for i in range(args.lat_samples):
+                torch.cuda.synchronize()
+                ts = time.perf_counter()
+                p32.execute_many_sparse([xs[i % S]], one_f, one_v)
+                lat.append((time.perf_counter() - ts) * 1e3)
+            return lat
+        lat_eager = latency()
+        p32.enable_graphs(True)
+        lat = latency()
+        p32.enable_graphs(False)
         result["latency_ms"] = {"p50": pct(lat, 50), "p99": pct(lat, 99),
-                                "samples": len(lat), "form": "fp32+refine, 1 signal"}
+                                "samples": len(lat),
+                                "form": "fp32+refine, 1 signal, replayed HIP graph",
+                                "eager_p50": pct(lat_eager, 50), "eager_p99": pct(lat_eager, 99)}
 
         # ------------------------------------------------------ FP64 forms
         p64 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp64", seed=(17, 4242))
+        p64.autotune(batch=min(S, 64))
         fr64, va64 = p64.make_sparse_outputs(S)
         p64.execute_many_sparse(xs, fr64, va64)  # warm-up
         torch.cuda.synchronize()

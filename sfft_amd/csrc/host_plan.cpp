@@ -546,7 +546,7 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   a.Hcap = P->v3_Hcap;
   a.Fcap = P->v3_Fcap;
   a.Rcap = P->v3_Rcap;
-  a.max_passes = 3 * 1024;
+  a.max_passes = 3 * 4096;  // the oracle's cap of the round-robin loop
   a.count = P->ws->count.as<int64_t>();
   a.status = P->ws->v3_status.as<int>();
   const bool want_sparse = io.freq != nullptr || io.val != nullptr;
