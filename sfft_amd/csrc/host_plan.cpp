@@ -51,6 +51,16 @@ int mod_inverse_int(int a, int n)
   return (int)v;
 }
 
+// The inverse of odd a modulo n = 2^p (the only n a plan accepts): Newton's
+// iteration x <- x (2 - a x) doubles the correct low bits, starting from
+// x = a (a*a = 1 mod 8).  The same unique value as mod_inverse_int.
+static inline int mod_inverse_pow2(int a, int n)
+{
+  uint32_t x = (uint32_t)a;
+  for (int i = 0; i < 4; i++) x *= 2u - (uint32_t)a * x;  // 3 -> 48 bits
+  return (int)(x & (uint32_t)(n - 1));
+}
+
 void Rng::seed(unsigned rs, long ds)
 {
   std::memset(&rd, 0, sizeof(rd));
@@ -456,9 +466,11 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   for (int s = 0; s < S; s++) {
     int *d = hp + (size_t)s * 6;
     Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
+    // gcd(a, n) == 1 (src/utils.cc:41-46) is "a odd" for n = 2^p; a = 0
+    // forces the first draw as gcd(0, n) = n does
     int a = 0;
-    while (gcd_int(a, n) != 1) a = (int)(R.next_random() % n);
-    const int ai = mod_inverse_int(a, n);
+    while ((a & 1) == 0) a = (int)(R.next_random() % n);
+    const int ai = mod_inverse_pow2(a, n);
     const int b = (int)(R.next_random() % n);
     d[0] = a;
     d[1] = ai;
@@ -684,9 +696,9 @@ int exec_batch(Plan *P, BatchIO &io)
     Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
     for (int l = 0; l < L; l++) {
       int a = 0;
-      while (gcd_int(a, n) != 1) a = (int)(R.next_random() % n);
+      while ((a & 1) == 0) a = (int)(R.next_random() % n);  // gcd(a, n) == 1, n = 2^p
       row[l] = a;
-      row[L + l] = mod_inverse_int(a, n);
+      row[L + l] = mod_inverse_pow2(a, n);
     }
     for (int c = 0; c < cl; c++)
       row[2 * L + c] = (int)(unsigned)std::floor(R.next_drand48() * sigma);

@@ -244,9 +244,10 @@ void worker_main(sfft_server *S, Worker *W)
     R.users++;
     const int64_t per = R.master->bytes_per_signal();
     auto fits = [&](size_t cnt) { return S->in_use() + (int64_t)cnt * per <= S->budget; };
+    // this worker's idle workspace counts as held: give it back first
+    if (!fits(batch.size()) && W->ws.device_bytes() > 0) W->ws.release();
     while (batch.size() > 1 && !fits(batch.size())) batch.pop_back();
     if (!fits(batch.size())) evict_for(S, (int64_t)batch.size() * per);
-    if (!fits(batch.size()) && S->running == 0) W->ws.release();
     if (!fits(batch.size())) {
       R.users--;
       if (S->running > 0) {  // running batches will give memory back
