@@ -402,7 +402,7 @@ Plan *clone_plan(const Plan *src, Workspace *ws, hipStream_t stream)
   P->v3_Fcap = src->v3_Fcap;
   P->v3_Rcap = src->v3_Rcap;
   P->v3_lds = src->v3_lds;
-  P->ws = ws;
+  P->ws = ws ? ws : &P->own_ws;
   return P;
 }
 

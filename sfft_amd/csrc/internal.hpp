@@ -181,7 +181,8 @@ int autotune_plan(Plan *P, int batch);
 // Applies a schedule tuned earlier in this process for P's key, if any.
 void apply_tuned_schedule(Plan *P);
 // A plan sharing src's device tables, running on `stream` with workspace ws
-// (both owned by the caller); src must outlive it.
+// (both owned by the caller; null ws: a workspace of its own); src must
+// outlive it.
 Plan *clone_plan(const Plan *src, Workspace *ws, hipStream_t stream);
 void free_plan(Plan *P);
 

@@ -556,10 +556,44 @@ __device__ __forceinline__ void sort_net(double (&v)[LMAX])
   }
 }
 
+// Ascending sort of v[O..O+10) with a 29-comparator 10-input network
+// (every 0-1 input checked).
+template <int LMAX, int O>
+__device__ __forceinline__ void sort10(double (&v)[LMAX])
+{
+  constexpr int net[29][2] = {{4, 9}, {3, 8}, {2, 7}, {1, 6}, {0, 5}, {1, 4}, {6, 9}, {0, 3},
+                              {5, 8}, {0, 2}, {3, 6}, {7, 9}, {0, 1}, {2, 4}, {5, 7}, {8, 9},
+                              {1, 2}, {4, 6}, {7, 8}, {3, 5}, {2, 5}, {6, 8}, {1, 3}, {4, 7},
+                              {2, 3}, {6, 7}, {3, 4}, {5, 6}, {4, 5}};
+#pragma unroll
+  for (int c = 0; c < 29; c++) {
+    const int lo = O + net[c][0], hi = O + net[c][1];
+    const double x = fmin(v[lo], v[hi]), y = fmax(v[lo], v[hi]);
+    v[lo] = x;
+    v[hi] = y;
+  }
+}
+
+// Lower median (rank 9) of 20 values: with both halves sorted, the 10th
+// smallest of their union is max_i min(A[i], B[9-i]).
+template <int LMAX>
+__device__ __forceinline__ double median20(double (&v)[LMAX])
+{
+  sort10<LMAX, 0>(v);
+  sort10<LMAX, 10>(v);
+  double m = fmin(v[0], v[19]);
+#pragma unroll
+  for (int i = 1; i < 10; i++) m = fmax(m, fmin(v[i], v[19 - i]));
+  return m;
+}
+
 template <int LMAX>
 __device__ __forceinline__ double2 lower_median(double (&vr)[LMAX], double (&vi)[LMAX],
                                                 int L)
 {
+  if constexpr (LMAX == 20) {
+    if (L == 20) return make_double2(median20<LMAX>(vr), median20<LMAX>(vi));
+  }
   sort_net<LMAX>(vr);
   sort_net<LMAX>(vi);
   const int rank = (L - 1) / 2;

@@ -65,10 +65,16 @@ struct Resident {
 struct Worker {
   std::thread th;
   hipStream_t stream = nullptr;
-  Workspace ws;
-  std::map<Key, Plan *> clones;
-  int64_t reserved = 0;  // workspace bytes the running batch may grow to
+  Workspace ws;                  // shared by the v1/v2 clones
+  std::map<Key, Plan *> clones;  // v3 clones hold their own workspace
+  int64_t reserved = 0;          // workspace bytes the running batch may grow to
   bool busy = false;
+  int64_t held() const
+  {
+    int64_t b = (int64_t)ws.device_bytes();
+    for (const auto &kv : clones) b += (int64_t)kv.second->own_ws.device_bytes();
+    return b;
+  }
 };
 
 }  // namespace
@@ -100,7 +106,7 @@ struct sfft_server {
   int64_t in_use() const
   {
     int64_t b = tables_bytes();
-    for (const Worker *w : workers) b += std::max<int64_t>((int64_t)w->ws.device_bytes(), w->reserved);
+    for (const Worker *w : workers) b += std::max<int64_t>(w->held(), w->reserved);
     return b;
   }
   void note_use()
@@ -232,24 +238,46 @@ void worker_main(sfft_server *S, Worker *W)
       S->cv.notify_all();
       continue;
     }
-    // the batch: queued requests of K in arrival order
+    // The batch, in arrival order: the queued requests of K, or, when K is a
+    // v3 key, the queued v3 requests of every key with resident tables (they
+    // run as one multi-plan launch).
+    const bool multi = K.version == (int)SFFT_VERSION_3;
     std::vector<Req *> batch;
     for (Req *q : S->queue) {
       if ((int)batch.size() >= S->max_batch) break;
-      if (q->key == K) batch.push_back(q);
+      if (q->key == K) {
+        batch.push_back(q);
+      } else if (multi && q->key.version == (int)SFFT_VERSION_3) {
+        auto it = S->plans.find(q->key);
+        if (it != S->plans.end() && it->second.master && !it->second.building)
+          batch.push_back(q);
+      }
     }
+    // pin every key of the batch (eviction skips pinned plans)
+    std::vector<Key> keys;
+    for (Req *q : batch)
+      if (std::find(keys.begin(), keys.end(), q->key) == keys.end()) {
+        keys.push_back(q->key);
+        S->plans[q->key].users++;
+      }
     // admission by HBM footprint: the batch's workspace is counted as new
-    // memory on top of what is held; cut the batch, then evict unused plans
-    // (never K: it is pinned by users), then wait for running batches
-    R.users++;
-    const int64_t per = R.master->bytes_per_signal();
-    auto fits = [&](size_t cnt) { return S->in_use() + (int64_t)cnt * per <= S->budget; };
-    // this worker's idle workspace counts as held: give it back first
-    if (!fits(batch.size()) && W->ws.device_bytes() > 0) W->ws.release();
-    while (batch.size() > 1 && !fits(batch.size())) batch.pop_back();
-    if (!fits(batch.size())) evict_for(S, (int64_t)batch.size() * per);
+    // memory on top of what is held; cut the batch, then evict unused plans,
+    // then wait for running batches
+    auto need = [&](size_t cnt) {
+      int64_t b = 0;
+      for (size_t i = 0; i < cnt; i++) b += S->plans[batch[i]->key].master->bytes_per_signal();
+      return b;
+    };
+    auto fits = [&](size_t cnt) { return S->in_use() + need(cnt) <= S->budget; };
+    // this worker's idle workspaces count as held: give them back first
     if (!fits(batch.size())) {
-      R.users--;
+      W->ws.release();
+      for (auto &kv : W->clones) kv.second->own_ws.release();
+    }
+    while (batch.size() > 1 && !fits(batch.size())) batch.pop_back();
+    if (!fits(batch.size())) evict_for(S, need(batch.size()));
+    if (!fits(batch.size())) {
+      for (const Key &k : keys) S->plans[k].users--;
       if (S->running > 0) {  // running batches will give memory back
         S->cv.wait(lk);
         continue;
@@ -260,57 +288,82 @@ void worker_main(sfft_server *S, Worker *W)
       S->cv.notify_all();
       continue;
     }
+    // unpin the keys the cut removed
+    for (auto it = keys.begin(); it != keys.end();) {
+      bool used = false;
+      for (Req *q : batch) used |= q->key == *it;
+      if (!used) {
+        S->plans[*it].users--;
+        it = keys.erase(it);
+      } else {
+        ++it;
+      }
+    }
     for (Req *q : batch) {
       S->queue.erase(std::find(S->queue.begin(), S->queue.end(), q));
       q->state = 1;
     }
-    R.last_use = ++S->tick;
-    W->reserved = (int64_t)W->ws.device_bytes() + (int64_t)batch.size() * per;
+    W->reserved = W->held() + need(batch.size());
     W->busy = true;
     S->running++;
     S->st.batches++;
     S->st.batched_requests += (int64_t)batch.size();
     S->st.max_concurrent_batches = std::max<int64_t>(S->st.max_concurrent_batches, S->running);
-    S->note_use();
-    Plan *C = nullptr;
-    auto ci = W->clones.find(K);
-    if (ci != W->clones.end()) {
-      C = ci->second;
-    } else {
-      C = clone_plan(R.master, &W->ws, W->stream);
-      W->clones[K] = C;
+    // one clone per key: v1/v2 clones share the worker's workspace, v3
+    // clones hold their own (a multi-plan launch keeps every key's state)
+    std::vector<Plan *> clones;
+    for (const Key &k : keys) {
+      Resident &Rk = S->plans[k];
+      Rk.last_use = ++S->tick;
+      auto ci = W->clones.find(k);
+      if (ci == W->clones.end())
+        ci = W->clones.emplace(k, clone_plan(Rk.master, multi ? nullptr : &W->ws, W->stream))
+                 .first;
+      clones.push_back(ci->second);
     }
+    S->note_use();
     lk.unlock();
 
-    const int nb = (int)batch.size();
-    std::vector<const double2 *> in(nb);
-    std::vector<int32_t *> fr(nb);
-    std::vector<double2 *> va(nb);
-    std::vector<Rng *> rng(nb);
-    std::vector<int64_t> counts(nb, 0);
-    bool any_f = false, any_v = false;
-    for (int i = 0; i < nb; i++) {
-      in[i] = reinterpret_cast<const double2 *>(batch[i]->r.in);
-      fr[i] = batch[i]->r.freq;
-      va[i] = reinterpret_cast<double2 *>(batch[i]->r.val);
-      rng[i] = &batch[i]->rng;
-      any_f |= fr[i] != nullptr;
-      any_v |= va[i] != nullptr;
+    // per key: its requests in arrival order
+    const size_t G = keys.size();
+    std::vector<std::vector<Req *>> grp(G);
+    for (Req *q : batch)
+      grp[std::find(keys.begin(), keys.end(), q->key) - keys.begin()].push_back(q);
+    std::vector<std::vector<const double2 *>> in(G);
+    std::vector<std::vector<int32_t *>> fr(G);
+    std::vector<std::vector<double2 *>> va(G);
+    std::vector<std::vector<Rng *>> rng(G);
+    std::vector<std::vector<int64_t>> counts(G);
+    std::vector<BatchIO> ios(G);
+    for (size_t g = 0; g < G; g++) {
+      bool any_out = false;
+      for (Req *q : grp[g]) {
+        in[g].push_back(reinterpret_cast<const double2 *>(q->r.in));
+        fr[g].push_back(q->r.freq);
+        va[g].push_back(reinterpret_cast<double2 *>(q->r.val));
+        rng[g].push_back(&q->rng);
+        any_out |= q->r.freq != nullptr || q->r.val != nullptr;
+      }
+      counts[g].assign(grp[g].size(), 0);
+      BatchIO &io = ios[g];
+      io.S = (int)grp[g].size();
+      io.rng = rng[g].data();
+      io.in = in[g].data();
+      io.freq = any_out ? fr[g].data() : nullptr;
+      io.val = any_out ? va[g].data() : nullptr;
+      io.cap = clones[g]->output_capacity();
+      io.counts = counts[g].data();
     }
-    BatchIO io;
-    io.S = nb;
-    io.rng = rng.data();
-    io.in = in.data();
-    io.freq = any_f ? fr.data() : nullptr;
-    io.val = any_v ? va.data() : nullptr;
-    io.cap = C->output_capacity();
-    io.counts = counts.data();
-    const int rc = exec_batch(C, io);
+    const int rc = multi ? exec_multi_v3(clones.data(), ios.data(), (int)G, W->stream, &W->ws)
+                         : exec_batch(clones[0], ios[0]);
     const std::string err = rc == 0 ? "" : std::string(last_error());
 
     lk.lock();
-    for (int i = 0; i < nb; i++) finish(S, batch[i], rc == 0, counts[i], err);
-    S->plans[K].users--;
+    for (size_t g = 0; g < G; g++) {
+      for (size_t i = 0; i < grp[g].size(); i++)
+        finish(S, grp[g][i], rc == 0, counts[g][i], err);
+      S->plans[keys[g]].users--;
+    }
     W->reserved = 0;
     W->busy = false;
     S->running--;

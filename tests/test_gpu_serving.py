@@ -104,3 +104,29 @@ def test_rejects_infeasible(torch_cuda):
     assert st["rejected"] == 1 and st["submitted"] == 0
     with pytest.raises(sfft_amd.SfftError):
         output_capacity(1 << 22, 1000, 2)
+
+
+def test_v3_keys_share_a_launch(torch_cuda):
+    """Queued v3 requests of different (n, k) run as one multi-plan launch
+    and give what each gives alone."""
+    torch = torch_cuda
+    keys = [(1 << 18, 50), (1 << 20, 100), (1 << 20, 300)]
+    xs = []
+    for g, (n, k) in enumerate(keys):
+        x, _, _ = signals_device(n, k, [40 + g, 50 + g])
+        xs += [(n, k, x[0].contiguous()), (n, k, x[1].contiguous())]
+    xv, _, _ = signals_device(1 << 22, 64, [9], snr_db=30.0)
+    with Server(workers=1, max_batch=16) as srv:
+        for n, k in keys:
+            srv.prepare(n, k, 3, "fp64")
+        srv.prepare(1 << 22, 64, 2, "fp32_refine")
+        tv = srv.submit(xv[0].contiguous(), 64, 2, "fp32_refine", (5, 6))
+        tk = [srv.submit(x, k, 3, "fp64", (100 + i, 200 + i)) for i, (n, k, x) in enumerate(xs)]
+        srv.wait(tv)
+        got = [srv.wait(t) for t in tk]
+        st = srv.stats()
+    assert st["completed"] == 7 and st["batches"] == 2
+    for i, (n, k, x) in enumerate(xs):
+        f1, v1 = _alone(torch, 3, n, k, "fp64", x, (100 + i, 200 + i))
+        assert np.array_equal(got[i][0].cpu().numpy(), f1)
+        assert np.array_equal(got[i][1].cpu().numpy(), v1)
