@@ -703,19 +703,19 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
   if (ALLFAST) {
     // every loop tabulated: a lean loop body with per-loop scalars in LDS
     for (int e = threadIdx.x; e < kTI * L; e += blockDim.x) {
-      const int il = e / L, l = e % L;
+      const int l = e / kTI, il = e % kTI;  // consecutive threads: consecutive il
       const int i = ti * kTI + il;
       if (i < nc) {
         int hashed;
         double2 G;
         hash_direct(a, l, s_ai[l], (unsigned)approved[i], hashed, G);
         const double den = G.x * G.x + G.y * G.y;
-        t_hash[il * LMAX + l] = hashed;
+        t_hash[l * kTI + il] = hashed;
         if (FP32) {
-          t_g[il * LMAX + l] = make_double2(G.x / den, G.y / den);
+          t_g[l * kTI + il] = make_double2(G.x / den, G.y / den);
         } else {
-          t_g[il * LMAX + l] = G;
-          t_den[il * LMAX + l] = den;
+          t_g[l * kTI + il] = G;
+          t_den[l * kTI + il] = den;
         }
       }
     }
@@ -724,6 +724,9 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
     const float2 *xs32 = FP32 ? a.xs32 + (int64_t)s * a.xs : nullptr;
     const bool conj = a.ref_conj;
     for (int e = threadIdx.x; e < kTI * kTJ; e += kBlock) {
+      // re-read the per-loop scalars from LDS every hit instead of holding
+      // 3 x LMAX of them in registers across the loop (occupancy)
+      asm volatile("" ::: "memory");
       const int jl = e / kTI;
       const int il = (e + jl) % kTI;  // skew: see below
       const int i = ti * kTI + il;
@@ -735,9 +738,9 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
       for (int l = 0; l < LMAX; l++) {
         if (l < L) {
           const int hashed =
-              (int)(((unsigned)t_hash[il * LMAX + l] + (unsigned)j * (s_step[l] & 0x7fffffffu)) &
+              (int)(((unsigned)t_hash[l * kTI + il] + (unsigned)j * (s_step[l] & 0x7fffffffu)) &
                     s_mask[l]);
-          const double2 g = t_g[il * LMAX + l];
+          const double2 g = t_g[l * kTI + il];
           double2 U;
           if (FP32) {
             const float2 u = xs32[s_off[l] + hashed];
@@ -748,7 +751,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
             vi[l] = conj ? q : -q;
           } else {
             U = xs64[s_off[l] + hashed];
-            const double den = t_den[il * LMAX + l];
+            const double den = t_den[l * kTI + il];
             vr[l] = (U.x * g.x + U.y * g.y) / den;
             const double q = (U.x * g.y - U.y * g.x) / den;
             vi[l] = conj ? q : -q;
@@ -763,19 +766,19 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
     return;
   }
   for (int e = threadIdx.x; e < kTI * L; e += blockDim.x) {
-    const int il = e / L, l = e % L;
+    const int l = e / kTI, il = e % kTI;  // consecutive threads: consecutive il
     const int i = ti * kTI + il;
     if (i < nc && (s_step[l] & 0x80000000u)) {
       int hashed;
       double2 G;
       hash_direct(a, l, s_ai[l], (unsigned)approved[i], hashed, G);
       const double den = G.x * G.x + G.y * G.y;
-      t_hash[il * LMAX + l] = hashed;
+      t_hash[l * kTI + il] = hashed;
       if (FP32) {
-        t_g[il * LMAX + l] = make_double2(G.x / den, G.y / den);
+        t_g[l * kTI + il] = make_double2(G.x / den, G.y / den);
       } else {
-        t_g[il * LMAX + l] = G;
-        t_den[il * LMAX + l] = den;
+        t_g[l * kTI + il] = G;
+        t_den[l * kTI + il] = den;
       }
     }
   }
@@ -801,17 +804,17 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
         const int64_t off = sbase + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
         const unsigned step = s_step[l];
         if (step & 0x80000000u) {
-          const int hashed = (t_hash[il * LMAX + l] +
+          const int hashed = (t_hash[l * kTI + il] +
                               (int)(((unsigned)j * (step & 0x7fffffffu)) & (unsigned)(B - 1))) &
                              (B - 1);
           const double2 U = load_bucket(a, FP32, isloc, off + hashed);
-          const double2 g = t_g[il * LMAX + l];
+          const double2 g = t_g[l * kTI + il];
           if (FP32) {
             // g = G / |G|^2: one complex multiply instead of two divisions
             vr[l] = U.x * g.x + U.y * g.y;
             vi[l] = a.ref_conj ? U.x * g.y - U.y * g.x : U.y * g.x - U.x * g.y;
           } else {
-            divide_by_filter(U, g, t_den[il * LMAX + l], a.ref_conj, vr[l], vi[l]);
+            divide_by_filter(U, g, t_den[l * kTI + il], a.ref_conj, vr[l], vi[l]);
           }
         } else {
           int hashed;
