@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -24,15 +25,49 @@ Plan *get(const sfft_plan *plan)
   return reinterpret_cast<Plan *>(plan->data);
 }
 
+// Device allocations seen recently: a batch's signals usually live in one
+// or a few allocations, so one runtime query serves them all (a query costs
+// microseconds; a batch has hundreds of pointers).
+struct DeviceRanges {
+  std::mutex mu;
+  uintptr_t lo[16] = {}, hi[16] = {};
+  int used = 0, next = 0;
+  bool find(uintptr_t a)
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (int i = 0; i < used; i++)
+      if (lo[i] <= a && a < hi[i]) return true;
+    return false;
+  }
+  void add(uintptr_t b, size_t size)
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    lo[next] = b;
+    hi[next] = b + size;
+    next = (next + 1) % 16;
+    used = std::min(used + 1, 16);
+  }
+} g_ranges;
+
 bool is_device(const void *p)
 {
   if (!p) return false;
+  if (g_ranges.find(reinterpret_cast<uintptr_t>(p))) return true;
   hipPointerAttribute_t attr;
   if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
-  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+  const bool dev = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+  if (dev) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && size)
+      g_ranges.add(reinterpret_cast<uintptr_t>(base), size);
+    else
+      (void)hipGetLastError();
+  }
+  return dev;
 }
 
 // Signals per GPU batch: bounded by workspace and, for host inputs, by the
