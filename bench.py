@@ -103,12 +103,12 @@ def main():
     cap = p32.capacity
     fr32, va32 = p32.make_sparse_outputs(S)
     for _ in range(args.warmup):
-        p32.execute_many_sparse(xs, fr32, va32)
+        p32.execute_many_sparse(x, fr32, va32)
     torch.cuda.synchronize()
     p32.enable_timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        counts32 = p32.execute_many_sparse(xs, fr32, va32)
+        counts32 = p32.execute_many_sparse(x, fr32, va32)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     stages32 = p32.stage_times()

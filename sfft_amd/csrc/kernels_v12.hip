@@ -665,15 +665,10 @@ __device__ __forceinline__ void write_result(const V12Launch &a, int s, int64_t 
 // so the filter value and base bucket are tabulated per (residue, loop).
 // Block tile: kTI residues x kTJ alias rows, kTI*kTJ/256 hits per thread.
 constexpr int kTI = 64, kTJ = 32;
-// Third dword of a gfx9-family buffer resource (raw 32-bit data, no
-// swizzle); out-of-range offsets read 0.
-constexpr int kBufferDword3 = 0x00020000;
 
-// FP32 buckets, tabulated loops: at least 3 wavefronts per SIMD (at most
-// 168 VGPRs) to hide the bucket loads' latency.
 template <int LMAX, bool FP32, bool ALLFAST>
-__global__ void __launch_bounds__(kBlock, FP32 && ALLFAST ? 3 : 1)
-    k_estimate_comb(V12Launch a, int tiles_i, int tiles_j)
+__global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles_i,
+                                                          int tiles_j)
 {
   __shared__ int t_hash[kTI * LMAX];
   __shared__ double2 t_g[kTI * LMAX];
@@ -682,7 +677,6 @@ __global__ void __launch_bounds__(kBlock, FP32 && ALLFAST ? 3 : 1)
   __shared__ unsigned s_step[LMAX];
   __shared__ unsigned s_mask[LMAX];
   __shared__ int s_off[LMAX];
-  __shared__ unsigned t_js[ALLFAST ? kTJ * LMAX : 1];
   int s;
   int64_t c;
   if (!xcd_block(blockIdx.x, a.S, (int64_t)tiles_i * tiles_j, s, c)) return;
@@ -726,74 +720,48 @@ __global__ void __launch_bounds__(kBlock, FP32 && ALLFAST ? 3 : 1)
       }
     }
     __syncthreads();
-    // The signal's buckets through a buffer resource (32-bit offsets: one
-    // add per load instead of 64-bit address arithmetic); per-loop step,
-    // mask and offset in scalar registers; outputs' base pointers hoisted.
-    const __amdgpu_buffer_rsrc_t rs =
-        FP32 ? __builtin_amdgcn_make_buffer_rsrc((void *)(a.xs32 + (int64_t)s * a.xs), 0,
-                                                 (int)(a.xs * 8), kBufferDword3)
-             : __builtin_amdgcn_make_buffer_rsrc((void *)(a.xs64 + (int64_t)s * a.xs), 0,
-                                                 (int)(a.xs * 16), kBufferDword3);
-    // (j * step) mod B of every (loop, row) of the tile: read with one
-    // address per wavefront (a row per wavefront)
-#pragma unroll
-    for (int l = 0; l < LMAX; l++)
-      if (l < L && threadIdx.x < kTJ)
-        t_js[l * kTJ + threadIdx.x] =
-            ((unsigned)(tj * kTJ + threadIdx.x) * (s_step[l] & 0x7fffffffu)) & s_mask[l];
-    __syncthreads();
-    int32_t *const out_f = a.o_freq ? a.o_freq[s] : nullptr;
-    double2 *const out_v = a.o_val ? a.o_val[s] : nullptr;
-    double2 *const out_d = a.o_dense ? a.o_dense[s] : nullptr;
+    const double2 *xs64 = a.xs64 + (int64_t)s * a.xs;
+    const float2 *xs32 = FP32 ? a.xs32 + (int64_t)s * a.xs : nullptr;
     const bool conj = a.ref_conj;
     for (int e = threadIdx.x; e < kTI * kTJ; e += kBlock) {
-      // re-read the tables from LDS every hit instead of holding LMAX of
-      // them in registers across the loop (occupancy)
+      // re-read the per-loop scalars from LDS every hit instead of holding
+      // 3 x LMAX of them in registers across the loop (occupancy)
       asm volatile("" ::: "memory");
-      const int jl = __builtin_amdgcn_readfirstlane(e / kTI);  // one row per wavefront
+      const int jl = e / kTI;
       const int il = (e + jl) % kTI;  // skew: see below
       const int i = ti * kTI + il;
       const int j = tj * kTJ + jl;
       if (i >= nc || j >= rows) continue;
       const unsigned h = (unsigned)j * (unsigned)a.W + (unsigned)approved[i];
-      // Branch-free over the LMAX slots (slots >= L repeat loop 0's load and
-      // become +inf pads), so all loads are issued before the first wait.
       double vr[LMAX], vi[LMAX];
 #pragma unroll
       for (int l = 0; l < LMAX; l++) {
-        const bool on = l < L;
-        const int ll = on ? l : 0;
-        const unsigned hashed =
-            ((unsigned)t_hash[ll * kTI + il] + t_js[ll * kTJ + jl]) & s_mask[ll];
-        const int off = s_off[ll] * (FP32 ? 8 : 16);
-        const double2 g = t_g[ll * kTI + il];
-        double r, q;
-        if (FP32) {
-          typedef unsigned v2u __attribute__((ext_vector_type(2)));
-          const v2u u = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(hashed * 8u) + off, 0, 0);
-          const double ux = (double)__uint_as_float(u.x), uy = (double)__uint_as_float(u.y);
-          // g = G / |G|^2: one complex multiply instead of two divisions
-          r = ux * g.x + uy * g.y;
-          q = ux * g.y - uy * g.x;
+        if (l < L) {
+          const int hashed =
+              (int)(((unsigned)t_hash[l * kTI + il] + (unsigned)j * (s_step[l] & 0x7fffffffu)) &
+                    s_mask[l]);
+          const double2 g = t_g[l * kTI + il];
+          double2 U;
+          if (FP32) {
+            const float2 u = xs32[s_off[l] + hashed];
+            U = make_double2((double)u.x, (double)u.y);
+            // g = G / |G|^2: one complex multiply instead of two divisions
+            vr[l] = U.x * g.x + U.y * g.y;
+            const double q = U.x * g.y - U.y * g.x;
+            vi[l] = conj ? q : -q;
+          } else {
+            U = xs64[s_off[l] + hashed];
+            const double den = t_den[l * kTI + il];
+            vr[l] = (U.x * g.x + U.y * g.y) / den;
+            const double q = (U.x * g.y - U.y * g.x) / den;
+            vi[l] = conj ? q : -q;
+          }
         } else {
-          typedef unsigned v4u __attribute__((ext_vector_type(4)));
-          const v4u u =
-              __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(hashed * 16u) + off, 0, 0);
-          const double ux = __hiloint2double((int)u.y, (int)u.x);
-          const double uy = __hiloint2double((int)u.w, (int)u.z);
-          const double den = t_den[ll * kTI + il];
-          r = (ux * g.x + uy * g.y) / den;
-          q = (ux * g.y - uy * g.x) / den;
+          vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
+          vi[l] = vr[l];
         }
-        const double pad = __longlong_as_double(0x7ff0000000000000ll);  // +inf
-        vr[l] = on ? r : pad;
-        vi[l] = on ? (conj ? q : -q) : pad;
       }
-      const double2 val = lower_median<LMAX>(vr, vi, L);
-      const int64_t t = (int64_t)j * nc + i;
-      if (out_f && t < a.o_cap) out_f[t] = (int32_t)h;
-      if (out_v && t < a.o_cap) out_v[t] = val;
-      if (out_d) out_d[h] = val;
+      write_result(a, s, (int64_t)j * nc + i, h, lower_median<LMAX>(vr, vi, L));
     }
     return;
   }
