@@ -419,8 +419,16 @@ void free_plan(Plan *P)
   }
   for (const auto &g : P->graphs) (void)hipGraphExecDestroy(g.exec);
   P->own_ws.release();
-  if (P->ev_made)
+  if (P->ev_made) {
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : P->ev_comb) (void)hipEventDestroy(e);
+  }
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  if (P->stream2) {
+    (void)hipStreamSynchronize(P->stream2);
+    (void)hipStreamDestroy(P->stream2);
+  }
   if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
 }
@@ -798,10 +806,17 @@ int exec_batch(Plan *P, BatchIO &io)
 
   if (P->timing && !P->ev_made) {
     for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
+    for (hipEvent_t &e : P->ev_comb) HIP_OK(hipEventCreate(&e));
     P->ev_made = true;
   }
+  if (p.comb && !P->stream2) {
+    HIP_OK(hipStreamCreateWithFlags(&P->stream2, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+  }
   int launches[5] = {0, 0, 0, 0, 0};
-  // the stream work of one batch: uploads, kernels, counts back
+  // The stream work of one batch: uploads, kernels, counts back.  The v2
+  // comb (its own gathers and FFTs) runs on stream2 beside the fold.
   auto enqueue = [&](bool timed) -> int {
     auto mark = [&](int i) {
       if (timed) (void)hipEventRecord(P->ev[i], st);
@@ -810,9 +825,16 @@ int exec_batch(Plan *P, BatchIO &io)
                           hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
                           hipMemcpyHostToDevice, st));
-    mark(0);
-    if (p.comb && k::comb(a, st) != 0) return -1;
-    launches[0] = p.comb ? (cl > 1 ? 2 : 1) : 0;
+    if (p.comb) {
+      hipStream_t st2 = P->stream2;
+      HIP_OK(hipEventRecord(P->ev_fork, st));
+      HIP_OK(hipStreamWaitEvent(st2, P->ev_fork, 0));
+      if (timed) (void)hipEventRecord(P->ev_comb[0], st2);
+      if (k::comb(a, st2) != 0) return -1;
+      if (timed) (void)hipEventRecord(P->ev_comb[1], st2);
+      HIP_OK(hipEventRecord(P->ev_join, st2));
+      launches[0] = cl > 1 ? 2 : 1;
+    }
     mark(1);
     if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
     mark(2);
@@ -820,6 +842,7 @@ int exec_batch(Plan *P, BatchIO &io)
     mark(3);
     if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
     launches[3] = p.comb ? 0 : 2;
+    if (p.comb) HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
     mark(4);
     if (k::estimate(a, st) != 0) return -1;
     launches[4] = 1;
@@ -871,7 +894,9 @@ int exec_batch(Plan *P, BatchIO &io)
   if (P->timing) {
     for (int i = 0; i < 5; i++) {
       float ms = 0;
-      if (launches[i] && hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]) == hipSuccess) {
+      const hipError_t e = i == 0 ? hipEventElapsedTime(&ms, P->ev_comb[0], P->ev_comb[1])
+                                  : hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]);
+      if (launches[i] && e == hipSuccess) {
         P->stage_ms[i] += ms;
         P->stage_launches[i] += launches[i];
       }

@@ -163,7 +163,12 @@ struct Plan {
   double stage_ms[kStages] = {0, 0, 0, 0, 0, 0};
   int64_t stage_launches[kStages] = {0, 0, 0, 0, 0, 0};
   hipEvent_t ev[kStages + 1] = {};
+  hipEvent_t ev_comb[2] = {};  // comb start / end on stream2 (timing)
   bool ev_made = false;
+  // The v2 comb runs on a second stream beside the fold and bucket FFTs;
+  // fork after the uploads, join before estimation.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   int64_t output_capacity() const;
   int64_t plan_bytes() const;

@@ -664,7 +664,7 @@ __device__ __forceinline__ void write_result(const V12Launch &a, int s, int64_t 
 // bucket centre and its bucket advances by (ai*W/m) mod B per alias row j,
 // so the filter value and base bucket are tabulated per (residue, loop).
 // Block tile: kTI residues x kTJ alias rows, kTI*kTJ/256 hits per thread.
-constexpr int kTI = 64, kTJ = 32;
+constexpr int kTI = 64, kTJ = 16;
 
 template <int LMAX, bool FP32, bool ALLFAST>
 __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles_i,
