@@ -136,21 +136,17 @@ def main():
     L = prm["loc_loops"] + prm["est_loops"]
     w_sum = prm["loc_loops"] * prm["w_loc"] + prm["est_loops"] * prm["w_est"]
     xs_size = prm["loc_loops"] * prm["B_loc"] + prm["est_loops"] * prm["B_est"]
-    # per step: gathered samples 16 B each, FP32 bucket writes, FP32 taps
-    # once per launch (a step runs the batch as a few pipelined chunks, one
-    # fold launch each)
+    # per launch (one batch): gathered samples 16 B each, FP32 taps once,
+    # FP32 bucket writes
+    fold_bytes = S * (16 * w_sum + 8 * xs_size) + 8 * (prm["w_loc"] + prm["w_est"])
     fold_ms, fold_launches = stages32["fold"]
-    per_step = max(1, fold_launches // args.steps)
-    fold_bytes_step = S * (16 * w_sum + 8 * xs_size) + per_step * 8 * (prm["w_loc"] + prm["w_est"])
-    fold_bytes = fold_bytes_step / per_step
     fold_avg_ms = fold_ms / max(1, fold_launches)
     achieved = fold_bytes / (fold_avg_ms * 1e-3) / 1e9
     kname = "k_fold_fft32" if schedule32[0] == 0 else "k_fold_stream<true>"
     result["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": None, "alg_bytes_per_launch": fold_bytes,
-                          "avg_launch_ms": fold_avg_ms, "launches": fold_launches,
-                          "signals_per_launch": S / per_step}
+                          "avg_launch_ms": fold_avg_ms, "launches": fold_launches}
     result["stages_ms_per_step"] = {s: v[0] / args.steps for s, v in stages32.items() if v[1]}
 
     if not args.only_fp32:

@@ -423,8 +423,6 @@ void free_plan(Plan *P)
     for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : P->ev_comb) (void)hipEventDestroy(e);
   }
-  for (hipEvent_t e : P->ev_chunk) (void)hipEventDestroy(e);
-  for (hipEvent_t e : P->ev_pipe) (void)hipEventDestroy(e);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
   if (P->stream2) {
@@ -660,30 +658,6 @@ int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Works
   return rc;
 }
 
-// The launch of signals [c0, c0 + cs) of a batch: every per-signal array
-// offset to its first signal.
-static k::V12Launch sub_launch(const k::V12Launch &a, int c0, int cs, int stride, bool comb)
-{
-  k::V12Launch b = a;
-  b.S = cs;
-  b.x = a.x + c0;
-  b.params = a.params + (int64_t)c0 * stride;
-  if (b.xs64) b.xs64 += (int64_t)c0 * a.xs;
-  if (b.xs32) b.xs32 += (int64_t)c0 * a.xs;
-  if (b.tmp64) b.tmp64 += (int64_t)c0 * a.xs;
-  if (b.tmp32) b.tmp32 += (int64_t)c0 * a.xs;
-  b.approved += (int64_t)c0 * std::max(a.comb_loops, 1) * a.num;
-  b.num_comb += c0;
-  b.J += (int64_t)c0 * a.Lloc * a.num;
-  if (!comb) b.hits += (int64_t)c0 * a.hits_cap;  // v2 keeps no vote list
-  if (b.mag_buf && !comb) b.mag_buf += (int64_t)c0 * a.Lloc * a.B_loc;
-  b.count += c0;
-  if (b.o_freq) b.o_freq += c0;
-  if (b.o_val) b.o_val += c0;
-  if (b.o_dense) b.o_dense += c0;
-  return b;
-}
-
 // Addresses a captured batch depends on (FNV-1a over the pointers).
 static uint64_t workspace_signature(const Workspace &w)
 {
@@ -835,30 +809,10 @@ int exec_batch(Plan *P, BatchIO &io)
     for (hipEvent_t &e : P->ev_comb) HIP_OK(hipEventCreate(&e));
     P->ev_made = true;
   }
-  // Large batches run as a pipeline of chunks: the fold (bound by L2-miss
-  // traffic) of chunk c+1 overlaps the selection and estimation (bound by
-  // L2 hits and arithmetic) of chunk c on a second stream.  Stage timing
-  // then sums per-chunk events: fold and bucket FFTs on the first stream,
-  // selection and estimation on the second.
-  const int nch = S >= 2 * kPipeMinChunk ? std::min(kPipeChunks, S / kPipeMinChunk) : 1;
-  if ((p.comb || nch > 1) && !P->stream2) {
+  if (p.comb && !P->stream2) {
     HIP_OK(hipStreamCreateWithFlags(&P->stream2, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
-  }
-  if ((int)P->ev_chunk.size() < nch) {
-    for (int c = (int)P->ev_chunk.size(); c < nch; c++) {
-      hipEvent_t e;
-      HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      P->ev_chunk.push_back(e);
-    }
-  }
-  if (P->timing && nch > 1 && (int)P->ev_pipe.size() < 5 * nch) {
-    for (int c = (int)P->ev_pipe.size(); c < 5 * nch; c++) {
-      hipEvent_t e;
-      HIP_OK(hipEventCreate(&e));
-      P->ev_pipe.push_back(e);
-    }
   }
   int launches[5] = {0, 0, 0, 0, 0};
   // The stream work of one batch: uploads, kernels, counts back.  The v2
@@ -871,55 +825,28 @@ int exec_batch(Plan *P, BatchIO &io)
                           hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
                           hipMemcpyHostToDevice, st));
-    hipStream_t st2 = P->stream2;
-    if (p.comb || nch > 1) {
+    if (p.comb) {
+      hipStream_t st2 = P->stream2;
       HIP_OK(hipEventRecord(P->ev_fork, st));
       HIP_OK(hipStreamWaitEvent(st2, P->ev_fork, 0));
-    }
-    if (p.comb) {
       if (timed) (void)hipEventRecord(P->ev_comb[0], st2);
       if (k::comb(a, st2) != 0) return -1;
       if (timed) (void)hipEventRecord(P->ev_comb[1], st2);
+      HIP_OK(hipEventRecord(P->ev_join, st2));
       launches[0] = cl > 1 ? 2 : 1;
     }
-    if (nch > 1) {
-      for (int c = 0; c < nch; c++) {
-        const int c0 = (int)((int64_t)S * c / nch), c1 = (int)((int64_t)S * (c + 1) / nch);
-        const k::V12Launch b = sub_launch(a, c0, c1 - c0, stride, p.comb);
-        hipEvent_t *pe = timed ? &P->ev_pipe[5 * c] : nullptr;
-        if (pe) (void)hipEventRecord(pe[0], st);
-        int f0, f1;
-        if ((f0 = k::fold_fft(b, st, 0)) < 0) return -1;
-        if (pe) (void)hipEventRecord(pe[1], st);
-        if ((f1 = k::fold_fft(b, st, 1)) < 0) return -1;
-        if (pe) (void)hipEventRecord(pe[2], st);
-        launches[1] += f0;
-        launches[2] += f1;
-        HIP_OK(hipEventRecord(P->ev_chunk[c], st));
-        HIP_OK(hipStreamWaitEvent(st2, P->ev_chunk[c], 0));
-        if (pe) (void)hipEventRecord(pe[3], st2);
-        if (!p.comb && k::select_vote_v1(b, st2) != 0) return -1;
-        if (k::estimate(b, st2) != 0) return -1;
-        if (pe) (void)hipEventRecord(pe[4], st2);
-        launches[4] += p.comb ? 1 : 3;
-      }
-      HIP_OK(hipEventRecord(P->ev_join, st2));
-      HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
-    } else {
-      if (p.comb) HIP_OK(hipEventRecord(P->ev_join, st2));
-      mark(1);
-      if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
-      mark(2);
-      if ((launches[2] = k::fold_fft(a, st, 1)) < 0) return -1;
-      mark(3);
-      if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
-      launches[3] = p.comb ? 0 : 2;
-      if (p.comb) HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
-      mark(4);
-      if (k::estimate(a, st) != 0) return -1;
-      launches[4] = 1;
-      mark(5);
-    }
+    mark(1);
+    if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
+    mark(2);
+    if ((launches[2] = k::fold_fft(a, st, 1)) < 0) return -1;
+    mark(3);
+    if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
+    launches[3] = p.comb ? 0 : 2;
+    if (p.comb) HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
+    mark(4);
+    if (k::estimate(a, st) != 0) return -1;
+    launches[4] = 1;
+    mark(5);
     HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
     return 0;
   };
@@ -930,7 +857,6 @@ int exec_batch(Plan *P, BatchIO &io)
     // batch of this shape; the workspace and staging addresses are part of
     // the key (kernel arguments and copy nodes hold them)
     const uint64_t key = (uint64_t)S | (uint64_t)fp32 << 32 | (uint64_t)want_sparse << 33 |
-                         (uint64_t)(nch > 1) << 63 |
                          (uint64_t)want_dense << 34 | (uint64_t)a.fold_mode << 35 |
                          (uint64_t)a.stream_blocks << 36;
     const uint64_t sig = workspace_signature(*P->ws) ^
@@ -965,26 +891,7 @@ int exec_batch(Plan *P, BatchIO &io)
     return -1;
   }
   HIP_OK(hipStreamSynchronize(st));
-  if (P->timing && nch > 1) {
-    float ms = 0;
-    if (launches[0] && hipEventElapsedTime(&ms, P->ev_comb[0], P->ev_comb[1]) == hipSuccess) {
-      P->stage_ms[0] += ms;
-      P->stage_launches[0] += launches[0];
-    }
-    // per chunk: fold, bucket FFTs (first stream), selection + estimation
-    const int pairs[3][3] = {{1, 0, 1}, {2, 1, 2}, {4, 3, 4}};
-    for (const auto &q : pairs) {
-      double tot = 0;
-      for (int c = 0; c < nch; c++)
-        if (hipEventElapsedTime(&ms, P->ev_pipe[5 * c + q[1]], P->ev_pipe[5 * c + q[2]]) ==
-            hipSuccess)
-          tot += ms;
-      if (launches[q[0]]) {
-        P->stage_ms[q[0]] += tot;
-        P->stage_launches[q[0]] += launches[q[0]];
-      }
-    }
-  } else if (P->timing) {
+  if (P->timing) {
     for (int i = 0; i < 5; i++) {
       float ms = 0;
       const hipError_t e = i == 0 ? hipEventElapsedTime(&ms, P->ev_comb[0], P->ev_comb[1])

@@ -20,10 +20,6 @@ using cd = std::complex<double>;
 constexpr int kMaxLdsFft64 = 8192;
 // Largest number of loops (location + estimation) the median network takes.
 constexpr int kMaxLoops = 32;
-// Pipelined batches: at most kPipeChunks chunks of at least kPipeMinChunk
-// signals.
-constexpr int kPipeChunks = 4;
-constexpr int kPipeMinChunk = 32;
 
 void set_error(const std::string &msg);
 
@@ -173,8 +169,6 @@ struct Plan {
   // fork after the uploads, join before estimation.
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  std::vector<hipEvent_t> ev_chunk;  // fold of chunk c done (pipelined batches)
-  std::vector<hipEvent_t> ev_pipe;   // 5 per chunk: stage timing of a pipeline
 
   int64_t output_capacity() const;
   int64_t plan_bytes() const;
