@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--schedule", default=None,
+                    help="fold schedule MODE,BLOCKS instead of autotuning (profiling runs)")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="Poisson arrival rate of the serving run, requests/s")
@@ -99,7 +101,11 @@ def main():
 
     # ---------------------------------------------------------- headline
     p32 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp32_refine", seed=(17, 4242))
-    schedule32 = p32.autotune(batch=min(S, 64))  # fold schedule only; same results
+    if args.schedule:  # fold schedule only; same results
+        p32.set_schedule(*[int(v) for v in args.schedule.split(",")])
+        schedule32 = p32.schedule()
+    else:
+        schedule32 = p32.autotune(batch=min(S, 64))
     cap = p32.capacity
     fr32, va32 = p32.make_sparse_outputs(S)
     for _ in range(args.warmup):
