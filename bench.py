@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--snr", type=float, default=40.0)
     ap.add_argument("--lat-samples", type=int, default=200)
     ap.add_argument("--plain-signals", type=int, default=64)
-    ap.add_argument("--cpu-signals", type=int, default=256)
+    ap.add_argument("--cpu-signals", type=int, default=2048)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
@@ -64,6 +64,23 @@ def parse():
                     help="first copy 1 GiB device-to-device (a kernel of known traffic "
                          "that scales the HBM byte counters)")
     return ap.parse_args()
+
+
+def measured_traffic(kernel, S, n, k):
+    """HBM bytes per launch of the fold from the committed rocprofv3 counter
+    runs (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), when this run
+    is the profiled workload; else None."""
+    if kernel != "k_fold_stream<true>" or (S, n, k) != (256, 1 << 22, 64):
+        return None
+    path = os.path.join(ROOT, "profiles", "traffic_v2_n22_k64_b256_fp32.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch k_fold_stream 256x2^22 k64:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def pct(a, q):
@@ -151,7 +168,8 @@ def main():
     kname = "k_fold_fft32" if schedule32[0] == 0 else "k_fold_stream<true>"
     result["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": None, "alg_bytes_per_launch": fold_bytes,
+                          "traffic": measured_traffic(kname, S, n, k),
+                          "alg_bytes_per_launch": fold_bytes,
                           "avg_launch_ms": fold_avg_ms, "launches": fold_launches}
     result["stages_ms_per_step"] = {s: v[0] / args.steps for s, v in stages32.items() if v[1]}
 
