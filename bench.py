@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
     ap.add_argument("--schedule", default=None,
                     help="fold schedule MODE,BLOCKS instead of autotuning (profiling runs)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the v1 / v2 single-signal and batched v3 configurations")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="Poisson arrival rate of the serving run, requests/s")
@@ -265,11 +267,16 @@ def main():
             "located_per_signal": int(counts32[0]),
         }
 
-    # ---------------------------------------------------------- serving
-    if not args.no_serve and not args.only_fp32:
+    # ---------------------------------------------------------- other configs
+    if not args.only_fp32 and not args.no_configs:
         del fr32, va32
         if "fr64" in locals():
             del fr64, va64
+        torch.cuda.empty_cache()
+        result["configs"] = other_configs(torch, dev)
+
+    # ---------------------------------------------------------- serving
+    if not args.no_serve and not args.only_fp32:
         torch.cuda.empty_cache()
         result["serving"] = serving(args, torch, dev)
 
@@ -277,6 +284,85 @@ def main():
     if not args.no_cpu and not args.only_fp32:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result), flush=True)
+
+
+def other_configs(torch, dev):
+    """The reference's other workloads, measured beside the headline:
+    plain v1 (n=2^20, k=50, noiseless) and v2 (n=2^22, k=100, SNR 30 dB)
+    single signals in the FP64 form, with per-coefficient L1 error against
+    the FP64 FFT of the signal; and batched v3, 64 signals of n=2^24 with k
+    cycling over {200, 500, 1000} (noiseless) in one multi-plan launch."""
+    import sfft_amd
+    from sfft_amd.sfft import execute_many_plans
+    from sfft_amd.signals import signals_device
+    out = {}
+    for name, (ver, n, k, snr) in {"v1_n20_k50": (1, 1 << 20, 50, None),
+                                   "v2_n22_k100_snr30": (2, 1 << 22, 100, 30.0)}.items():
+        x, fs, vs = signals_device(n, k, [77], snr_db=snr, device=dev)
+        p64 = sfft_amd.sfft(n, k, ver, strict=False, precision="fp64", seed=(17, 4242))
+        p32 = sfft_amd.sfft(n, k, ver, strict=False, precision="fp32_refine", seed=(17, 4242))
+        d = torch.empty(n, dtype=torch.complex128, device=dev)
+        p64.execute_device(x[0], d)
+        torch.cuda.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            p64.execute_device(x[0], d)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        f64, v64 = p64.make_sparse_outputs(1)
+        f32, v32 = p32.make_sparse_outputs(1)
+        p64.seed(17, 4242)
+        p32.seed(17, 4242)
+        c64 = int(p64.execute_many_sparse(x, f64, v64)[0])
+        c32 = int(p32.execute_many_sparse(x, f32, v32)[0])
+        Xf = torch.fft.fft(x[0]) / n
+        loc = f64[0, :c64].long()
+        l1_loc = float((v64[0, :c64] - Xf[loc].conj()).abs().mean())
+        tf = torch.from_numpy(fs[0]).to(dev)
+        got = torch.zeros(n, dtype=torch.complex128, device=dev)
+        got[loc] = v64[0, :c64]
+        l1_true = float((got[tf] - Xf[tf].conj()).abs().mean())
+        same = c32 == c64 and torch.equal(f32[0, :c32], f64[0, :c64])
+        out[name] = {"fp64_ms_per_transform": ms, "located": c64,
+                     "l1_vs_fp64_fft_located": l1_loc, "l1_vs_fp64_fft_true_support": l1_true,
+                     "located_set_diff_fp32_vs_fp64": 0 if same else 1,
+                     "value_maxabs_fp32_vs_fp64": float((v32[0, :c32] - v64[0, :c64]).abs().max())
+                     if same else None}
+        del x, d, Xf, got
+        torch.cuda.empty_cache()
+    # batched v3, uneven k in one launch
+    n, S = 1 << 24, 64
+    ks = [200, 500, 1000]
+    plans = {kk: sfft_amd.sfft(n, kk, 3, seed=(17 + kk, 4242)) for kk in ks}
+    xs, truth = [], []
+    for i in range(S):
+        kk = ks[i % 3]
+        x, fs, vs = signals_device(n, kk, [9000 + i], device=dev)
+        xs.append(x[0])
+        truth.append((fs[0], vs[0]))
+    order = [plans[ks[i % 3]] for i in range(S)]
+    outs = [(torch.empty(p.capacity, dtype=torch.int32, device=dev),
+             torch.empty(p.capacity, dtype=torch.complex128, device=dev)) for p in order]
+    execute_many_plans(order, xs, outs)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = execute_many_plans(order, xs, outs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    found, err = [], 0.0
+    for (fr, va), (fs, vs) in zip(res, truth):
+        got = dict(zip(fr.cpu().numpy().tolist(), va.cpu().numpy().tolist()))
+        hit = [int(f) in got for f in fs]
+        found.append(float(np.mean(hit)))
+        err = max([err] + [abs(got[int(f)] - v) for f, v in zip(fs, vs) if int(f) in got])
+    out["v3_n24_b64_k200_500_1000"] = {
+        "transforms_per_s": S / dt, "ms_per_batch": dt * 1e3, "form": "fp64, one launch",
+        "recovered_fraction_by_k": {str(kk): float(np.mean(found[j::3])) for j, kk in enumerate(ks)},
+        "max_value_error_recovered": err}
+    return out
 
 
 def serving(args, torch, dev):
