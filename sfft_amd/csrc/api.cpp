@@ -373,8 +373,8 @@ int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks)
 {
   Plan *P = get(plan);
   if (!P) return -1;
-  if ((fold_mode != 0 && fold_mode != 1) || stream_blocks < 0 || stream_blocks > (1 << 20)) {
-    set_error("fold_mode must be 0 or 1 and stream_blocks in [0, 2^20]");
+  if (fold_mode < 0 || fold_mode > 2 || stream_blocks < 0 || stream_blocks > (1 << 20)) {
+    set_error("fold_mode must be 0, 1 or 2 and stream_blocks in [0, 2^20]");
     return -1;
   }
   P->fold_mode = fold_mode;

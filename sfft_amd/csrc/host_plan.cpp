@@ -307,7 +307,7 @@ Plan *make_plan(int n, int k, sfft_version v)
     return P;
   }
   // tuning overrides for experiments: fold schedule and its resident blocks
-  if (const char *e = std::getenv("SFFT_FOLD_MODE")) P->fold_mode = std::atoi(e) ? 1 : 0;
+  if (const char *e = std::getenv("SFFT_FOLD_MODE")) P->fold_mode = std::min(2, std::max(0, std::atoi(e)));
   if (const char *e = std::getenv("SFFT_FOLD_BLOCKS")) P->stream_blocks = std::atoi(e);
   P->v12 = derive_v12(n, k, v == SFFT_VERSION_2);
   std::string why = check_v12(P->v12);
@@ -795,7 +795,13 @@ int exec_batch(Plan *P, BatchIO &io)
   a.tmp64 = big64 ? P->ws->tmp64.as<double2>() : nullptr;
   a.tmp32 = big32 ? P->ws->tmp32.as<float2>() : nullptr;
   // the fused fold + FFT keeps a whole bucket array in LDS
-  a.fold_mode = std::max(p.B_loc, p.B_est) > kMaxLdsFft64 ? 1 : P->fold_mode;
+  a.fold_mode = std::max(p.B_loc, p.B_est) > kMaxLdsFft64 ? std::max(1, P->fold_mode)
+                                                          : P->fold_mode;
+  // alias-row bucket layout (schedule 1): v2 FP32, W a multiple of n/B for
+  // both loop kinds, at least one wavefront of alias rows, FFTs in LDS
+  a.rows_layout = fp32 && p.comb && a.fold_mode == 1 &&
+                  p.W_comb % (n / p.B_loc) == 0 && p.W_comb % (n / p.B_est) == 0 &&
+                  n / p.W_comb >= 64 && std::max(p.B_loc, p.B_est) <= 2 * kMaxLdsFft64;
   a.stream_blocks = P->stream_blocks;
   if (a.stream_blocks <= 0) {
     int dev = 0, cus = 256;

@@ -244,6 +244,8 @@ struct V12Launch {
   int64_t est_cap;          // upper bound of the located count per signal
   int fold_mode;            // 0: fused fold+FFT per (signal, loop) block;
                             // 1: signal-major streamed fold, then bucket FFTs
+                            //    (alias-row layout where it applies);
+                            // 2: as 1, buckets always in natural order
   int stream_blocks;        // resident blocks of the streamed fold
   // scratch for transforms and selections that do not fit in LDS (null
   // when unused)
@@ -251,6 +253,9 @@ struct V12Launch {
   double *mag_buf;               // |.|^2 of comb or v1 selection arrays
   double2 *tmp64;                // S x xs: radix-2 pass copy of xs64
   float2 *tmp32;                 // S x xs: radix-2 pass copy of xs32
+  // v2 FP32 with W a multiple of n/B: buckets after the FFT stored by
+  // alias row (see k_bucket_fft_rows); estimation reads them contiguously
+  bool rows_layout;
 };
 // Largest comb transform / selection array kept in LDS.
 constexpr int kMaxLdsComb = 8192;

@@ -839,6 +839,122 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
   }
 }
 
+// ------------------------------------------------- v2 FP32: alias-row layout
+// With W a multiple of m = n/B, frequency h = j*W + c of comb residue c
+// lies in bucket hashed0(c) + j*ai*d (mod B) of a loop, d = W/m, for its
+// alias rows j < rows = n/W = B/d.  Stored at position r*rows + (q * u^-1
+// mod rows) for bucket b = r + d*q (u = ai mod rows, odd), the buckets of
+// rows j, j+1, ... of one residue are adjacent: a wavefront of 64 rows reads
+// 512 contiguous bytes instead of 64 scattered lines.  The products and the
+// median are the same as in k_estimate_comb, so the results are identical.
+
+// inverse of odd a modulo 2^32 (Newton's iteration from a*a = 1 mod 8)
+__device__ __forceinline__ unsigned inv_odd(unsigned a)
+{
+  unsigned x = a;
+#pragma unroll
+  for (int i = 0; i < 4; i++) x *= 2u - a * x;
+  return x;
+}
+
+// Bucket FFTs of FP32 loops [l0, l0 + nl), written in the alias-row layout.
+__global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0, int nl)
+{
+  extern __shared__ float2 rbuf[];
+  const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
+  const int B = l < a.Lloc ? a.B_loc : a.B_est;
+  const int logB = ilog2(B);
+  const int rows = a.n / a.W, d = B / rows;
+  const int stride = 2 * a.L + max(a.comb_loops, 1);
+  const unsigned ai = (unsigned)a.params[(int64_t)s * stride + a.L + l];
+  const unsigned uinv = inv_odd(ai) & (unsigned)(rows - 1);
+  float2 *io = a.xs32 + (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+  for (int m = threadIdx.x; m < B; m += blockDim.x) rbuf[bitrev(m, logB)] = io[m];
+  __syncthreads();
+  lds_fft(rbuf, logB, a.tw32, a.tw_nmax / B);
+  for (int b = threadIdx.x; b < B; b += blockDim.x) {
+    const int r = b % d, q = b / d;
+    io[r * rows + (int)(((unsigned)q * uinv) & (unsigned)(rows - 1))] = rbuf[b];
+  }
+}
+
+constexpr int kRI = 32;  // residues per tile of k_estimate_rows (8 per wavefront)
+
+template <int LMAX>
+__global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles_i, int tiles_j)
+{
+  __shared__ int t_rb[kRI * LMAX];       // r * rows of hashed0
+  __shared__ int t_q[kRI * LMAX];        // q0 * u^-1 mod rows
+  __shared__ double2 t_g[kRI * LMAX];    // G / |G|^2
+  __shared__ int s_off[LMAX];
+  __shared__ double2 stage[64 * (kRI + 1)];  // [row][residue], padded
+  int s;
+  int64_t c;
+  if (!xcd_block(blockIdx.x, a.S, (int64_t)tiles_i * tiles_j, s, c)) return;
+  const int ti = (int)(c % tiles_i), tj = (int)(c / tiles_i);
+  const int nc = a.num_comb[s];
+  const int rows = a.n / a.W;
+  if (ti * kRI >= nc) return;
+  const int L = a.L;
+  const int stride = 2 * L + max(a.comb_loops, 1);
+  const int *approved = a.approved + (int64_t)s * a.comb_loops * a.num;
+  if (threadIdx.x < L) s_off[threadIdx.x] = (int)loop_offset(threadIdx.x, a.Lloc, a.B_loc, a.B_est);
+  for (int e = threadIdx.x; e < kRI * L; e += blockDim.x) {
+    const int l = e / kRI, il = e % kRI;
+    const int i = ti * kRI + il;
+    if (i < nc) {
+      const unsigned ai = (unsigned)a.params[(int64_t)s * stride + L + l];
+      int hashed;
+      double2 G;
+      hash_direct(a, l, ai, (unsigned)approved[i], hashed, G);
+      const int B = l < a.Lloc ? a.B_loc : a.B_est;
+      const int d = B / rows;
+      const unsigned uinv = inv_odd(ai) & (unsigned)(rows - 1);
+      const double den = G.x * G.x + G.y * G.y;
+      t_rb[l * kRI + il] = (hashed % d) * rows;
+      t_q[l * kRI + il] = (int)(((unsigned)(hashed / d) * uinv) & (unsigned)(rows - 1));
+      t_g[l * kRI + il] = make_double2(G.x / den, G.y / den);
+    }
+  }
+  __syncthreads();
+  const float2 *xs32 = a.xs32 + (int64_t)s * a.xs;
+  const bool conj = a.ref_conj;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int j = tj * 64 + lane;             // one alias row per lane
+  const int jm = j & (rows - 1);
+  for (int q = 0; q < kRI / 4; q++) {
+    asm volatile("" ::: "memory");
+    const int il = wid * (kRI / 4) + q;     // one residue per wavefront
+    if (ti * kRI + il >= nc) break;         // wave-uniform; no barrier inside
+    double vr[LMAX], vi[LMAX];
+#pragma unroll
+    for (int l = 0; l < LMAX; l++) {
+      if (l < L) {
+        const int pos = t_rb[l * kRI + il] + ((t_q[l * kRI + il] + jm) & (rows - 1));
+        const float2 u = xs32[s_off[l] + pos];
+        const double2 g = t_g[l * kRI + il];
+        const double ux = (double)u.x, uy = (double)u.y;
+        vr[l] = ux * g.x + uy * g.y;
+        const double qq = ux * g.y - uy * g.x;
+        vi[l] = conj ? qq : -qq;
+      } else {
+        vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
+        vi[l] = vr[l];
+      }
+    }
+    stage[lane * (kRI + 1) + il] = lower_median<LMAX>(vr, vi, L);
+  }
+  __syncthreads();
+  // row by row: kRI consecutive output slots per row
+  for (int e = threadIdx.x; e < 64 * kRI; e += blockDim.x) {
+    const int jr = e / kRI, il = e % kRI;
+    const int i = ti * kRI + il, jj = tj * 64 + jr;
+    if (i < nc && jj < rows)
+      write_result(a, s, (int64_t)jj * nc + i, (unsigned)jj * (unsigned)a.W + (unsigned)approved[i],
+                   stage[jr * (kRI + 1) + il]);
+  }
+}
+
 // v1: one thread per hit of the vote list.
 template <int LMAX, bool FP32>
 __global__ void __launch_bounds__(kBlock) k_estimate_hits(V12Launch a, int64_t chunks)
@@ -948,7 +1064,7 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part)
   const int B64 = n64 == 0 ? 0 : (n64 > a.Lloc ? max(a.B_loc, a.B_est) : a.B_loc);
   const int B32 = n64 >= a.L ? 0 : (n64 == 0 ? max(a.B_loc, a.B_est) : a.B_est);
   int launches = 0;
-  if (a.fold_mode == 1) {
+  if (a.fold_mode >= 1) {
     const dim3 blk(64 * kFoldWaves);
     if (part == 0) {
       if (n64 > 0) {
@@ -967,6 +1083,14 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part)
         if (launched("fold_stream32") != 0) return -1;
         launches++;
       }
+    } else if (a.rows_layout) {
+      // v2 FP32: every loop FP32, written in the alias-row layout
+      const int Bm = max(a.B_loc, a.B_est);
+      allow_lds(k_bucket_fft_rows, (size_t)Bm * sizeof(float2));
+      hipLaunchKernelGGL(k_bucket_fft_rows, dim3(a.S * a.L), dim3(kBlock),
+                         (size_t)Bm * sizeof(float2), st, a, 0, a.L);
+      if (launched("bucket_fft_rows") != 0) return -1;
+      launches++;
     } else {
       // the location and estimation loops as two batches (their bucket
       // counts may differ): in LDS up to the LDS size, else radix-2 runs
@@ -1028,7 +1152,12 @@ int select_vote_v1(const V12Launch &a, hipStream_t st)
 template <int LMAX>
 void launch_estimate(const V12Launch &a, hipStream_t st)
 {
-  if (a.comb_loops > 0) {
+  if (a.comb_loops > 0 && a.rows_layout) {
+    const int tiles_i = (a.comb_loops * a.num + kRI - 1) / kRI;
+    const int tiles_j = (a.n / a.W + 63) / 64;
+    const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));
+    hipLaunchKernelGGL((k_estimate_rows<LMAX>), grid, dim3(kBlock), 0, st, a, tiles_i, tiles_j);
+  } else if (a.comb_loops > 0) {
     const int tiles_i = (a.comb_loops * a.num + kTI - 1) / kTI;
     const int tiles_j = (a.n / a.W + kTJ - 1) / kTJ;
     const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));

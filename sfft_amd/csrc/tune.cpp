@@ -80,6 +80,7 @@ int autotune_plan(Plan *P, int batch)
   std::vector<std::pair<int, int>> cand;
   if (std::max(p.B_loc, p.B_est) <= kMaxLdsFft64) cand.push_back({0, 0});
   for (int r : {1, 2, 4}) cand.push_back({1, r * cus});
+  if (P->precision == SFFT_PRECISION_FP32_REFINE && p.comb) cand.push_back({2, cus});
   const int mode0 = P->fold_mode, blocks0 = P->stream_blocks;
   double best = 1e30;
   std::pair<int, int> pick{mode0, blocks0};

@@ -181,11 +181,12 @@ def test_host_pointers_dense(torch_cuda):
 @pytest.mark.parametrize("version,n,k,snr", [(2, 1 << 20, 64, 30.0), (1, 1 << 18, 100, 30.0)])
 def test_schedules_give_identical_results(torch_cuda, version, n, k, snr, precision):
     """Fused and streamed folds (any resident-block count) sum every bucket
-    in the same tap order: identical bits."""
+    in the same tap order, and the alias-row bucket layout (schedule 1, v2
+    FP32) feeds estimation the same products: identical bits."""
     print(f"case v{version} n={n} k={k} {precision}", flush=True)
     x, f, v = signal_np(n, k, seed=6000 + k, snr_db=snr)
     res = []
-    for mode, blocks in [(0, 0), (1, 0), (1, 96), (1, 1024)]:
+    for mode, blocks in [(0, 0), (1, 0), (1, 96), (1, 1024), (2, 0)]:
         plan = _gpu_plan(version, n, k, precision)
         plan.set_schedule(mode, blocks)
         assert plan.schedule() == (mode, blocks)
