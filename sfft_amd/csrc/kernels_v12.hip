@@ -880,7 +880,9 @@ __global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0,
 
 constexpr int kRI = 32;  // residues per tile of k_estimate_rows (8 per wavefront)
 
-template <int LMAX>
+// FULL: L == LMAX, so the loop over loops has no branch and the compiler
+// issues all bucket loads of a frequency before the first wait.
+template <int LMAX, bool FULL>
 __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles_i, int tiles_j)
 {
   __shared__ int t_rb[kRI * LMAX];       // r * rows of hashed0
@@ -929,7 +931,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
     double vr[LMAX], vi[LMAX];
 #pragma unroll
     for (int l = 0; l < LMAX; l++) {
-      if (l < L) {
+      if (FULL || l < L) {
         const int pos = t_rb[l * kRI + il] + ((t_q[l * kRI + il] + jm) & (rows - 1));
         const float2 u = xs32[s_off[l] + pos];
         const double2 g = t_g[l * kRI + il];
@@ -942,7 +944,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
         vi[l] = vr[l];
       }
     }
-    stage[lane * (kRI + 1) + il] = lower_median<LMAX>(vr, vi, L);
+    stage[lane * (kRI + 1) + il] = lower_median<LMAX>(vr, vi, FULL ? LMAX : L);
   }
   __syncthreads();
   // row by row: kRI consecutive output slots per row
@@ -1156,7 +1158,12 @@ void launch_estimate(const V12Launch &a, hipStream_t st)
     const int tiles_i = (a.comb_loops * a.num + kRI - 1) / kRI;
     const int tiles_j = (a.n / a.W + 63) / 64;
     const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));
-    hipLaunchKernelGGL((k_estimate_rows<LMAX>), grid, dim3(kBlock), 0, st, a, tiles_i, tiles_j);
+    if (a.L == LMAX)
+      hipLaunchKernelGGL((k_estimate_rows<LMAX, true>), grid, dim3(kBlock), 0, st, a, tiles_i,
+                         tiles_j);
+    else
+      hipLaunchKernelGGL((k_estimate_rows<LMAX, false>), grid, dim3(kBlock), 0, st, a, tiles_i,
+                         tiles_j);
   } else if (a.comb_loops > 0) {
     const int tiles_i = (a.comb_loops * a.num + kTI - 1) / kTI;
     const int tiles_j = (a.n / a.W + kTJ - 1) / kTJ;
