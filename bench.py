@@ -215,14 +215,17 @@ def main():
         counts64 = p64.execute_many_sparse(xs, fr64, va64)
         counts32 = p32.execute_many_sparse(xs, fr32, va32)
         torch.cuda.synchronize()
-        set_diff, vmax = 0, 0.0
+        set_diff, vmax, l1_sum, l1_cnt = 0, 0.0, 0.0, 0
         for i in range(S):
             c32, c64 = int(counts32[i]), int(counts64[i])
             a, b = fr32[i, :c32], fr64[i, :c64]
             if c32 != c64 or not torch.equal(a, b):
                 set_diff += 1
                 continue
-            vmax = max(vmax, float((va32[i, :c32] - va64[i, :c64]).abs().max()))
+            dv = (va32[i, :c32] - va64[i, :c64]).abs()
+            vmax = max(vmax, float(dv.max()))
+            l1_sum += float(dv.sum())
+            l1_cnt += c32
 
         # plain FP64 form: one sfft_exec per signal into a dense device output
         plain = sfft_amd.sfft(n, k, 2, strict=False, precision="fp64", seed=(17, 4242))
@@ -262,6 +265,7 @@ def main():
         result["accuracy"] = {
             "located_set_diff_fp32_vs_fp64": set_diff,
             "value_maxabs_fp32_vs_fp64": vmax,
+            "l1_fp32_vs_fp64_per_coefficient": l1_sum / max(1, l1_cnt),
             "l1_vs_fp64_fft_located_set": float(np.mean(l1_loc)),
             "l1_vs_fp64_fft_true_support": float(np.mean(l1_true)),
             "located_per_signal": int(counts32[0]),
