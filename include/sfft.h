@@ -131,9 +131,9 @@ int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision);
  * src/computefourier-1.0-2.0.cc:222-275 onto the GPU).  fold_mode 0: one
  * workgroup per (signal, loop) folds into LDS and transforms there; 1: a
  * signal-major streamed fold with stream_blocks resident workgroups (0: twice
- * the CU count), then the bucket FFTs, which in the v2 FP32 form store the
- * buckets by alias row so that estimation reads them contiguously; 2: as 1
- * with the buckets in natural order.  Schedules never change results.
+ * the CU count), then the bucket FFTs, which for v2 store the buckets by
+ * alias row so that estimation reads them contiguously; 2: as 1 with the
+ * buckets in natural order.  Schedules never change results.
  * Returns 0 or -1. */
 int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks);
 int sfft_plan_get_schedule(const sfft_plan *plan, int *fold_mode, int *stream_blocks);
@@ -206,9 +206,8 @@ int sfft_plan_stage_times(const sfft_plan *plan, double *ms6,
 /* Diagnostics: the bucket values (after the bucket FFTs) of signal `signal`
  * of the plan's last v1/v2 batch, all loops concatenated as the reference's
  * x_samp (src/computefourier-1.0-2.0.cc:228-230): count entries into host
- * memory.  In the v2 FP32 form under schedule 1 each loop's buckets are in
- * the alias-row order (see sfft_plan_set_schedule).  Returns the number
- * written or -1. */
+ * memory, in natural bucket order whatever the schedule's storage order.
+ * Returns the number written or -1. */
 int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
                            int64_t count);
 

@@ -560,6 +560,26 @@ int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
   }
   for (size_t i = 0; i < t64.size(); i++) dst[i] = complex_t(t64[i].x, t64[i].y);
   for (size_t i = 0; i < t32.size(); i++) dst[n64 + i] = complex_t(t32[i].x, t32[i].y);
+  if (P->last_rows_layout && m == xs) {
+    // back to natural bucket order: bucket b = r + d*q of loop l was stored
+    // at r*rows + (q * u^-1 mod rows), u = ai mod rows (the batch's draws
+    // are still in the staging buffer)
+    const V12Params &p = P->v12;
+    const int L = p.loops(), stride = 2 * L + std::max(p.comb_loops, 1);
+    const int *hp = reinterpret_cast<const int *>(P->ws->pinned);
+    const int rows = p.n / p.W_comb;
+    std::vector<complex_t> tmp;
+    int64_t off = 0;
+    for (int l = 0; l < L; l++) {
+      const int B = l < p.loc_loops ? p.B_loc : p.B_est, d = B / rows;
+      uint32_t ai = (uint32_t)hp[(size_t)signal * stride + L + l], inv = ai;
+      for (int it = 0; it < 4; it++) inv *= 2u - ai * inv;
+      tmp.assign(dst + off, dst + off + B);
+      for (int b = 0; b < B; b++)
+        dst[off + b] = tmp[(size_t)(b % d) * rows + (((uint32_t)(b / d) * inv) & (uint32_t)(rows - 1))];
+      off += B;
+    }
+  }
   return m;
 }
 

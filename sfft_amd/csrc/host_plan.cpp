@@ -797,11 +797,12 @@ int exec_batch(Plan *P, BatchIO &io)
   // the fused fold + FFT keeps a whole bucket array in LDS
   a.fold_mode = std::max(p.B_loc, p.B_est) > kMaxLdsFft64 ? std::max(1, P->fold_mode)
                                                           : P->fold_mode;
-  // alias-row bucket layout (schedule 1): v2 FP32, W a multiple of n/B for
-  // both loop kinds, at least one wavefront of alias rows, FFTs in LDS
-  a.rows_layout = fp32 && p.comb && a.fold_mode == 1 &&
-                  p.W_comb % (n / p.B_loc) == 0 && p.W_comb % (n / p.B_est) == 0 &&
-                  n / p.W_comb >= 64 && std::max(p.B_loc, p.B_est) <= 2 * kMaxLdsFft64;
+  // alias-row bucket layout (schedule 1): v2, W a multiple of n/B for both
+  // loop kinds, at least one wavefront of alias rows, FFTs in LDS
+  a.rows_layout = p.comb && a.fold_mode == 1 && p.W_comb % (n / p.B_loc) == 0 &&
+                  p.W_comb % (n / p.B_est) == 0 && n / p.W_comb >= 64 &&
+                  std::max(p.B_loc, p.B_est) <= (fp32 ? 2 : 1) * kMaxLdsFft64;
+  P->last_rows_layout = a.rows_layout;
   a.stream_blocks = P->stream_blocks;
   if (a.stream_blocks <= 0) {
     int dev = 0, cus = 256;

@@ -118,6 +118,7 @@ struct Plan {
   int fold_mode = 1;       // see k::V12Launch::fold_mode
   int stream_blocks = 0;   // 0: derived from the CU count
   bool last_fp32 = false;  // precision of the buckets in ws_xsamp
+  bool last_rows_layout = false;  // the last batch stored buckets by alias row
   bool ref_conj = true;
   Rng rng;
   hipStream_t stream = nullptr;
@@ -253,8 +254,8 @@ struct V12Launch {
   double *mag_buf;               // |.|^2 of comb or v1 selection arrays
   double2 *tmp64;                // S x xs: radix-2 pass copy of xs64
   float2 *tmp32;                 // S x xs: radix-2 pass copy of xs32
-  // v2 FP32 with W a multiple of n/B: buckets after the FFT stored by
-  // alias row (see k_bucket_fft_rows); estimation reads them contiguously
+  // v2 with W a multiple of n/B: buckets after the FFT stored by alias row
+  // (see k_bucket_fft_rows); estimation reads them contiguously
   bool rows_layout;
 };
 // Largest comb transform / selection array kept in LDS.

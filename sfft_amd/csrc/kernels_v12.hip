@@ -839,7 +839,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
   }
 }
 
-// ------------------------------------------------- v2 FP32: alias-row layout
+// ------------------------------------------------------ v2: alias-row layout
 // With W a multiple of m = n/B, frequency h = j*W + c of comb residue c
 // lies in bucket hashed0(c) + j*ai*d (mod B) of a loop, d = W/m, for its
 // alias rows j < rows = n/W = B/d.  Stored at position r*rows + (q * u^-1
@@ -857,10 +857,13 @@ __device__ __forceinline__ unsigned inv_odd(unsigned a)
   return x;
 }
 
-// Bucket FFTs of FP32 loops [l0, l0 + nl), written in the alias-row layout.
+// Bucket FFTs of loops [l0, l0 + nl), written in the alias-row layout.
+template <bool FP32>
 __global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0, int nl)
 {
-  extern __shared__ float2 rbuf[];
+  using T2 = typename std::conditional<FP32, float2, double2>::type;
+  extern __shared__ char rraw[];
+  T2 *rbuf = reinterpret_cast<T2 *>(rraw);
   const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
   const int B = l < a.Lloc ? a.B_loc : a.B_est;
   const int logB = ilog2(B);
@@ -868,10 +871,13 @@ __global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0,
   const int stride = 2 * a.L + max(a.comb_loops, 1);
   const unsigned ai = (unsigned)a.params[(int64_t)s * stride + a.L + l];
   const unsigned uinv = inv_odd(ai) & (unsigned)(rows - 1);
-  float2 *io = a.xs32 + (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
+  T2 *io;
+  if constexpr (FP32) io = a.xs32; else io = a.xs64;
+  io += (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est);
   for (int m = threadIdx.x; m < B; m += blockDim.x) rbuf[bitrev(m, logB)] = io[m];
   __syncthreads();
-  lds_fft(rbuf, logB, a.tw32, a.tw_nmax / B);
+  if constexpr (FP32) lds_fft(rbuf, logB, a.tw32, a.tw_nmax / B);
+  else lds_fft(rbuf, logB, a.tw64, a.tw_nmax / B);
   for (int b = threadIdx.x; b < B; b += blockDim.x) {
     const int r = b % d, q = b / d;
     io[r * rows + (int)(((unsigned)q * uinv) & (unsigned)(rows - 1))] = rbuf[b];
@@ -882,12 +888,13 @@ constexpr int kRI = 32;  // residues per tile of k_estimate_rows (8 per wavefron
 
 // FULL: L == LMAX, so the loop over loops has no branch and the compiler
 // issues all bucket loads of a frequency before the first wait.
-template <int LMAX, bool FULL>
+template <int LMAX, bool FULL, bool FP32>
 __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles_i, int tiles_j)
 {
   __shared__ int t_rb[kRI * LMAX];       // r * rows of hashed0
   __shared__ int t_q[kRI * LMAX];        // q0 * u^-1 mod rows
-  __shared__ double2 t_g[kRI * LMAX];    // G / |G|^2
+  __shared__ double2 t_g[kRI * LMAX];    // FP32 form: G / |G|^2; FP64: G
+  __shared__ double t_den[FP32 ? 1 : kRI * LMAX];  // FP64: |G|^2
   __shared__ int s_off[LMAX];
   __shared__ double2 stage[64 * (kRI + 1)];  // [row][residue], padded
   int s;
@@ -915,11 +922,17 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
       const double den = G.x * G.x + G.y * G.y;
       t_rb[l * kRI + il] = (hashed % d) * rows;
       t_q[l * kRI + il] = (int)(((unsigned)(hashed / d) * uinv) & (unsigned)(rows - 1));
-      t_g[l * kRI + il] = make_double2(G.x / den, G.y / den);
+      if (FP32) {
+        t_g[l * kRI + il] = make_double2(G.x / den, G.y / den);
+      } else {
+        t_g[l * kRI + il] = G;
+        t_den[l * kRI + il] = den;
+      }
     }
   }
   __syncthreads();
-  const float2 *xs32 = a.xs32 + (int64_t)s * a.xs;
+  const float2 *xs32 = FP32 ? a.xs32 + (int64_t)s * a.xs : nullptr;
+  const double2 *xs64 = FP32 ? nullptr : a.xs64 + (int64_t)s * a.xs;
   const bool conj = a.ref_conj;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int j = tj * 64 + lane;             // one alias row per lane
@@ -933,12 +946,17 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
     for (int l = 0; l < LMAX; l++) {
       if (FULL || l < L) {
         const int pos = t_rb[l * kRI + il] + ((t_q[l * kRI + il] + jm) & (rows - 1));
-        const float2 u = xs32[s_off[l] + pos];
         const double2 g = t_g[l * kRI + il];
-        const double ux = (double)u.x, uy = (double)u.y;
-        vr[l] = ux * g.x + uy * g.y;
-        const double qq = ux * g.y - uy * g.x;
-        vi[l] = conj ? qq : -qq;
+        if (FP32) {
+          const float2 u = xs32[s_off[l] + pos];
+          const double ux = (double)u.x, uy = (double)u.y;
+          vr[l] = ux * g.x + uy * g.y;
+          const double qq = ux * g.y - uy * g.x;
+          vi[l] = conj ? qq : -qq;
+        } else {
+          // the reference's operation order (cf12 383-398), as k_estimate_comb
+          divide_by_filter(xs64[s_off[l] + pos], g, t_den[l * kRI + il], conj, vr[l], vi[l]);
+        }
       } else {
         vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
         vi[l] = vr[l];
@@ -1086,11 +1104,17 @@ int fold_fft(const V12Launch &a, hipStream_t st, int part)
         launches++;
       }
     } else if (a.rows_layout) {
-      // v2 FP32: every loop FP32, written in the alias-row layout
+      // v2: every loop in one precision, written in the alias-row layout
       const int Bm = max(a.B_loc, a.B_est);
-      allow_lds(k_bucket_fft_rows, (size_t)Bm * sizeof(float2));
-      hipLaunchKernelGGL(k_bucket_fft_rows, dim3(a.S * a.L), dim3(kBlock),
-                         (size_t)Bm * sizeof(float2), st, a, 0, a.L);
+      if (a.fp32) {
+        allow_lds(k_bucket_fft_rows<true>, (size_t)Bm * sizeof(float2));
+        hipLaunchKernelGGL(k_bucket_fft_rows<true>, dim3(a.S * a.L), dim3(kBlock),
+                           (size_t)Bm * sizeof(float2), st, a, 0, a.L);
+      } else {
+        allow_lds(k_bucket_fft_rows<false>, (size_t)Bm * sizeof(double2));
+        hipLaunchKernelGGL(k_bucket_fft_rows<false>, dim3(a.S * a.L), dim3(kBlock),
+                           (size_t)Bm * sizeof(double2), st, a, 0, a.L);
+      }
       if (launched("bucket_fft_rows") != 0) return -1;
       launches++;
     } else {
@@ -1158,12 +1182,21 @@ void launch_estimate(const V12Launch &a, hipStream_t st)
     const int tiles_i = (a.comb_loops * a.num + kRI - 1) / kRI;
     const int tiles_j = (a.n / a.W + 63) / 64;
     const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));
-    if (a.L == LMAX)
-      hipLaunchKernelGGL((k_estimate_rows<LMAX, true>), grid, dim3(kBlock), 0, st, a, tiles_i,
-                         tiles_j);
-    else
-      hipLaunchKernelGGL((k_estimate_rows<LMAX, false>), grid, dim3(kBlock), 0, st, a, tiles_i,
-                         tiles_j);
+    if (a.fp32) {
+      if (a.L == LMAX)
+        hipLaunchKernelGGL((k_estimate_rows<LMAX, true, true>), grid, dim3(kBlock), 0, st, a,
+                           tiles_i, tiles_j);
+      else
+        hipLaunchKernelGGL((k_estimate_rows<LMAX, false, true>), grid, dim3(kBlock), 0, st, a,
+                           tiles_i, tiles_j);
+    } else {
+      if (a.L == LMAX)
+        hipLaunchKernelGGL((k_estimate_rows<LMAX, true, false>), grid, dim3(kBlock), 0, st, a,
+                           tiles_i, tiles_j);
+      else
+        hipLaunchKernelGGL((k_estimate_rows<LMAX, false, false>), grid, dim3(kBlock), 0, st, a,
+                           tiles_i, tiles_j);
+    }
   } else if (a.comb_loops > 0) {
     const int tiles_i = (a.comb_loops * a.num + kTI - 1) / kTI;
     const int tiles_j = (a.n / a.W + kTJ - 1) / kTJ;
