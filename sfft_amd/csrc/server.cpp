@@ -232,7 +232,9 @@ void worker_main(sfft_server *S, Worker *W)
         R2.table_bytes = P->plan_bytes();
         R2.last_use = ++S->tick;
         S->st.plans_created++;
+        R2.users++;  // never evict the plan just built for its own requests
         evict_for(S, 0);
+        R2.users--;
         S->note_use();
       }
       S->cv.notify_all();
@@ -455,7 +457,9 @@ int sfft_server_prepare(sfft_server *S, int n, int k, sfft_version version,
     R.table_bytes = P->plan_bytes();
     R.last_use = ++S->tick;
     S->st.plans_created++;
+    R.users++;  // never evict the plan just built
     evict_for(S, 0);
+    R.users--;
     S->note_use();
   } else {
     S->plans.erase(K);
