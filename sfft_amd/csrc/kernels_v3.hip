@@ -143,11 +143,33 @@ __device__ void gauss_removal(const Ctx &c, int key, double2 value, int B,
   }
 }
 
-__device__ void gauss_apply(double2 *S0, double2 *S1, int B, int h, const double2 *r)
+// Pair i subtracts r[0..2] from buckets h, h+1, h-1 of S0 and r[3..5] from
+// those of S1 (in that order; with B <= 2 a bucket can be several of them).
+// Thread t owns the buckets b = t mod blockDim and takes the pairs in pair
+// order, 8 bucket indices loaded ahead.
+__device__ void gauss_apply_owned(double2 *S0, double2 *S1, int B, const int *remi, int slot,
+                                  const double2 *rem, int roff, int cnt)
 {
-  const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
-  for (int q = 0; q < 3; q++) S0[bk[q]] = csub(S0[bk[q]], r[q]);
-  for (int q = 0; q < 3; q++) S1[bk[q]] = csub(S1[bk[q]], r[3 + q]);
+  const int T = blockDim.x, t = threadIdx.x;
+  if (t >= B) return;
+  for (int i0 = 0; i0 < cnt; i0 += 8) {
+    int hh[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) hh[u] = i0 + u < cnt ? remi[(i0 + u) * 3 + slot] : -1;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int h = hh[u];
+      if (h < 0) break;
+      const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        if (bk[q] % T != t) continue;
+        const double2 *r = rem + (int64_t)(i0 + u) * 14 + roff;
+        S0[bk[q]] = csub(S0[bk[q]], r[q]);
+        S1[bk[q]] = csub(S1[bk[q]], r[3 + q]);
+      }
+    }
+  }
 }
 
 // update_mansour_loops2 (cf3 300-351)
@@ -182,20 +204,28 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
                     a.win2_half, wmul(c.aa, c.gG), 1, ri[1], r + 6);
     if (which & 4) mansour_removal(c, key, v, ri[2], r + 12);
   }
+  __threadfence_block();
   __syncthreads();
-  // one lane per bucket array, in pair order
-  if (threadIdx.x == 0 && (which & 1)) {
-    for (int i = 0; i < cnt; i++)
-      gauss_apply(c.GS0, c.GS1, c.B1, c.remi[i * 3], c.rem + (int64_t)i * 14);
-  } else if (threadIdx.x == 64 && (which & 2)) {
-    for (int i = 0; i < cnt; i++)
-      gauss_apply(c.GP0, c.GP1, c.B2, c.remi[i * 3 + 1], c.rem + (int64_t)i * 14 + 6);
-  } else if (threadIdx.x == 128 && (which & 4)) {
-    for (int i = 0; i < cnt; i++) {
-      const int h = c.remi[i * 3 + 2];
-      const double2 *r = c.rem + (int64_t)i * 14 + 12;
-      c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
-      c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
+  // One thread per bucket, each taking the pairs in pair order: a bucket
+  // sees the same subtractions in the same order as when one thread applies
+  // every pair in turn (cf3 465-482), so the results are the same.
+  if (which & 1) gauss_apply_owned(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt);
+  if (which & 2) gauss_apply_owned(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt);
+  if (which & 4) {
+    const int T = blockDim.x, t = threadIdx.x;
+    for (int i0 = 0; i0 < cnt; i0 += 8) {
+      int hh[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) hh[u] = i0 + u < cnt ? c.remi[(i0 + u) * 3 + 2] : -1;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int h = hh[u];
+        if (h < 0) break;
+        if (h % T != t) continue;
+        const double2 *r = c.rem + (int64_t)(i0 + u) * 14 + 12;
+        c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
+        c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
+      }
     }
   }
   __threadfence_block();
