@@ -85,6 +85,24 @@ def measured_traffic(kernel, S, n, k):
     return None
 
 
+def copy_rate(torch, dev):
+    """Device-to-device copy of 2 GiB (read + written bytes / time): the
+    achievable HBM rate beside the vendor peak."""
+    src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 5 * 2 * src.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
@@ -173,6 +191,7 @@ def main():
                           "traffic": measured_traffic(kname, S, n, k),
                           "alg_bytes_per_launch": fold_bytes,
                           "avg_launch_ms": fold_avg_ms, "launches": fold_launches}
+    result["roofline"]["copy_gbs"] = copy_rate(torch, dev)
     result["stages_ms_per_step"] = {s: v[0] / args.steps for s, v in stages32.items() if v[1]}
 
     if not args.only_fp32:
@@ -215,7 +234,7 @@ def main():
         counts64 = p64.execute_many_sparse(xs, fr64, va64)
         counts32 = p32.execute_many_sparse(xs, fr32, va32)
         torch.cuda.synchronize()
-        set_diff, vmax, l1_sum, l1_cnt = 0, 0.0, 0.0, 0
+        set_diff, vmax, vrel, l1_sum, l1_cnt = 0, 0.0, 0.0, 0.0, 0
         for i in range(S):
             c32, c64 = int(counts32[i]), int(counts64[i])
             a, b = fr32[i, :c32], fr64[i, :c64]
@@ -224,8 +243,15 @@ def main():
                 continue
             dv = (va32[i, :c32] - va64[i, :c64]).abs()
             vmax = max(vmax, float(dv.max()))
+            vrel = max(vrel, float(dv.max() / va64[i, :c64].abs().max().clamp_min(1e-300)))
             l1_sum += float(dv.sum())
             l1_cnt += c32
+        # signal 0 with the first draws of seed (17, 4242), for the CPU
+        # oracle's located-set check in the CPU leg
+        c0 = int(counts64[0])
+        gpu_check = {"x": xs[0].cpu().numpy(), "filters": (p64.filter(0), p64.filter(1)),
+                     "freqs": fr64[0, :c0].cpu().numpy(), "vals": va64[0, :c0].cpu().numpy(),
+                     "freqs32": fr32[0, :int(counts32[0])].cpu().numpy()}
 
         # plain FP64 form: one sfft_exec per signal into a dense device output
         plain = sfft_amd.sfft(n, k, 2, strict=False, precision="fp64", seed=(17, 4242))
@@ -252,12 +278,18 @@ def main():
         # ------------------------------------------------------ accuracy
         # FP64 full FFT of the signal as the oracle; the reference's v1/v2
         # values are conj(X), so compare with conj.
-        l1_loc, l1_true = [], []
+        l1_loc, l1_true, off_energy = [], [], []
         for i in range(min(4, S)):
             Xf = torch.fft.fft(xs[i]) / n
             c = int(counts32[i])
             f_idx = fr32[i, :c].long()
             l1_loc.append(float((va32[i, :c] - Xf[f_idx].conj()).abs().mean()))
+            # energy the output puts off the true support, relative to the
+            # support's energy
+            on = torch.isin(f_idx, torch.as_tensor(np.asarray(fs[i]), device=dev).long())
+            e_off = float((va32[i, :c][~on].abs() ** 2).sum())
+            e_sup = float((Xf[torch.as_tensor(np.asarray(fs[i]), device=dev).long()].abs() ** 2).sum())
+            off_energy.append(e_off / max(e_sup, 1e-300))
             pos = {int(ff): j for j, ff in enumerate(fr32[i, :c].cpu().numpy())}
             tv = [abs(complex(va32[i, pos[int(ff)]].item()) - np.conj(complex(Xf[int(ff)].item())))
                   for ff in fs[i] if int(ff) in pos]
@@ -265,6 +297,8 @@ def main():
         result["accuracy"] = {
             "located_set_diff_fp32_vs_fp64": set_diff,
             "value_maxabs_fp32_vs_fp64": vmax,
+            "value_max_rel_to_peak_fp32_vs_fp64": vrel,
+            "off_support_energy_ratio": float(np.mean(off_energy)),
             "l1_fp32_vs_fp64_per_coefficient": l1_sum / max(1, l1_cnt),
             "l1_vs_fp64_fft_located_set": float(np.mean(l1_loc)),
             "l1_vs_fp64_fft_true_support": float(np.mean(l1_true)),
@@ -286,7 +320,7 @@ def main():
 
     # ---------------------------------------------------------- CPU
     if not args.no_cpu and not args.only_fp32:
-        result["cpu_baseline"] = cpu_baseline(args)
+        result["cpu_baseline"] = cpu_baseline(args, locals().get("gpu_check"))
     print(json.dumps(result), flush=True)
 
 
@@ -450,7 +484,29 @@ def serving(args, torch, dev):
     return out
 
 
-def cpu_baseline(args):
+def oracle_check(check, n, k):
+    """Located set and values of the FP64 GPU form on one signal against the
+    oracle run on the plan's filters with the same draws (bit-exact by
+    construction: same arithmetic, same order)."""
+    import oracle
+    (lt, lf), (et, ef) = check["filters"]
+    oracle.seed_glibc(17, 4242)
+    P = oracle.V12(n, k, True)
+    P.set_filters(lt, lf, et, ef)
+    a, ai, off = P.draw()
+    tr = P.exec(check["x"], a, ai, off, trace=True)
+    located = np.unique(tr["hits"][:tr["hits_prefill"]])
+    fr = check["freqs"]
+    return {"signal": "batch entry 0, first draws of seed (17, 4242)",
+            "located": int(len(located)),
+            "located_set_diff_fp64_vs_oracle": int(len(np.setxor1d(np.unique(fr), located))),
+            "located_set_diff_fp32_vs_oracle":
+                int(len(np.setxor1d(np.unique(check["freqs32"]), located))),
+            "value_maxabs_fp64_vs_oracle": float(np.abs(check["vals"] - tr["out"][fr]).max())
+            if len(fr) else 0.0}
+
+
+def cpu_baseline(args, check=None):
     """The CPU oracle (restatement of the reference) on this host's cores."""
     import ctypes as C
     import oracle
@@ -484,12 +540,15 @@ def cpu_baseline(args):
                     break
     except OSError:
         pass
-    return {"value": num / dt, "unit": "transforms/s", "cores": threads, "kind": "port",
+    out = {"value": num / dt, "unit": "transforms/s", "cores": threads, "kind": "port",
             "sample": "%d execs (%d distinct signals) of v2 n=2^%d k=%d SNR %g dB; "
                       "reference semantics incl. the dense zero-filled output; "
                       "OpenMP over signals; %.1f s wall"
                       % (num, distinct, int(np.log2(n)), k, args.snr, dt),
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
+    if check is not None:
+        out["parity"] = oracle_check(check, n, k)
+    return out
 
 
 if __name__ == "__main__":

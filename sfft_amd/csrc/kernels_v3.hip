@@ -23,6 +23,9 @@ using namespace sfa::dev;
 namespace {
 
 constexpr int kV3Block = 512;
+// Subtraction passes of up to this many pairs run on one thread per bucket
+// array; longer ones on bucket-owner threads.
+constexpr int kSerialApply = 128;
 
 __device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
 
@@ -143,6 +146,13 @@ __device__ void gauss_removal(const Ctx &c, int key, double2 value, int B,
   }
 }
 
+__device__ void gauss_apply_pair(double2 *S0, double2 *S1, int B, int h, const double2 *r)
+{
+  const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
+  for (int q = 0; q < 3; q++) S0[bk[q]] = csub(S0[bk[q]], r[q]);
+  for (int q = 0; q < 3; q++) S1[bk[q]] = csub(S1[bk[q]], r[3 + q]);
+}
+
 // Pair i subtracts r[0..2] from buckets h, h+1, h-1 of S0 and r[3..5] from
 // those of S1 (in that order; with B <= 2 a bucket can be several of them).
 // Thread t owns the buckets b = t mod blockDim and takes the pairs in pair
@@ -209,6 +219,25 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
   // One thread per bucket, each taking the pairs in pair order: a bucket
   // sees the same subtractions in the same order as when one thread applies
   // every pair in turn (cf3 465-482), so the results are the same.
+  if (cnt <= kSerialApply) {
+    // few pairs: one thread per bucket array applies them all
+    if (threadIdx.x == 0 && (which & 1)) {
+      for (int i = 0; i < cnt; i++) gauss_apply_pair(c.GS0, c.GS1, c.B1, c.remi[i * 3], c.rem + (int64_t)i * 14);
+    } else if (threadIdx.x == 64 && (which & 2)) {
+      for (int i = 0; i < cnt; i++)
+        gauss_apply_pair(c.GP0, c.GP1, c.B2, c.remi[i * 3 + 1], c.rem + (int64_t)i * 14 + 6);
+    } else if (threadIdx.x == 128 && (which & 4)) {
+      for (int i = 0; i < cnt; i++) {
+        const int h = c.remi[i * 3 + 2];
+        const double2 *r = c.rem + (int64_t)i * 14 + 12;
+        c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
+        c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    return;
+  }
   if (which & 1) gauss_apply_owned(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt);
   if (which & 2) gauss_apply_owned(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt);
   if (which & 4) {
