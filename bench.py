@@ -60,6 +60,9 @@ def parse():
                     help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=16)
+    ap.add_argument("--only-configs", action="store_true",
+                    help="run only the v1 / v2 single-signal and batched v3 configurations "
+                         "(for profilers)")
     ap.add_argument("--only-fp32", action="store_true",
                     help="time only the headline form (for profilers)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -125,6 +128,9 @@ def main():
     n, k, S = args.n, args.k, args.batch
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.only_configs:
+        print(json.dumps({"configs": other_configs(torch, dev)}), flush=True)
+        return
     if args.calib_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

@@ -200,6 +200,7 @@ int64_t Plan::bytes_per_signal() const
   if (fp32 && Bm > 2 * kMaxLdsFft64) b += xs * 8;
   if (p.comb && p.W_comb > k::kMaxLdsComb) b += (int64_t)p.comb_loops * p.W_comb * 40;
   if (!p.comb && p.B_loc > k::kMaxLdsSelect) b += (int64_t)p.loc_loops * p.B_loc * 8;
+  if (!p.comb) b += 4LL * k::vote_chunks(n, p.B_loc, p.B_thresh, p.loc_loops, p.thr_loops);
   return b + (int64_t)p.comb_loops * p.B_thresh * 4 + (int64_t)p.loc_loops * p.B_thresh * 4 +
          output_capacity() * (4 + 16 + 4) + (2 * p.loops() + p.comb_loops + 8) * 4;
 }
@@ -711,6 +712,7 @@ int exec_batch(Plan *P, BatchIO &io)
     for (int c = 0; c < cl; c++)
       row[2 * L + c] = (int)(unsigned)std::floor(R.next_drand48() * sigma);
   }
+  const int vchunks = p.comb ? 0 : k::vote_chunks(n, p.B_loc, num, p.loc_loops, p.thr_loops);
   // v1 keeps its location loops in FP64 under fp32 (selection-critical)
   const bool loc64 = fp32 && !p.comb;
   const bool need64 = !fp32 || loc64;
@@ -730,7 +732,7 @@ int exec_batch(Plan *P, BatchIO &io)
       !P->ws->numcomb.ensure((size_t)S * 4) ||
       !P->ws->J.ensure((size_t)S * p.loc_loops * num * 4) ||
       !P->ws->hits.ensure((size_t)S * (p.comb ? 1 : cap) * 4) ||
-      !P->ws->count.ensure((size_t)S * 8))
+      !P->ws->count.ensure((size_t)S * 8 + (p.comb ? 0 : (size_t)S * vchunks * 4)))
     return -1;
   // scratch for the sizes above the LDS limits
   const bool big_comb = p.comb && p.W_comb > k::kMaxLdsComb;
@@ -784,6 +786,7 @@ int exec_batch(Plan *P, BatchIO &io)
   a.hits = P->ws->hits.as<int>();
   a.hits_cap = cap;
   a.count = P->ws->count.as<int64_t>();
+  a.vote_cnt = p.comb ? nullptr : reinterpret_cast<int *>(a.count + S);
   a.o_freq = want_sparse ? reinterpret_cast<int32_t *const *>(dtab + tab_bytes) : nullptr;
   a.o_val = want_sparse ? reinterpret_cast<double2 *const *>(dtab + 2 * tab_bytes) : nullptr;
   a.o_dense = want_dense ? reinterpret_cast<double2 *const *>(dtab + 3 * tab_bytes) : nullptr;
@@ -848,7 +851,7 @@ int exec_batch(Plan *P, BatchIO &io)
     if ((launches[2] = k::fold_fft(a, st, 1)) < 0) return -1;
     mark(3);
     if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
-    launches[3] = p.comb ? 0 : 2;
+    launches[3] = p.comb ? 0 : 4;
     if (p.comb) HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
     mark(4);
     if (k::estimate(a, st) != 0) return -1;
@@ -900,6 +903,9 @@ int exec_batch(Plan *P, BatchIO &io)
   HIP_OK(hipStreamSynchronize(st));
   if (P->timing) {
     for (int i = 0; i < 5; i++) {
+      // only stages that ran: an event never recorded fails the query, and
+      // that error would be reported by the next launch check
+      if (!launches[i]) continue;
       float ms = 0;
       const hipError_t e = i == 0 ? hipEventElapsedTime(&ms, P->ev_comb[0], P->ev_comb[1])
                                   : hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]);

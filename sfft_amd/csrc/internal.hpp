@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <cstdlib>
@@ -236,6 +237,7 @@ struct V12Launch {
   int *num_comb;   // S
   int *J;          // S x Lloc x num
   int *hits;       // S x hits_cap
+  int *vote_cnt;   // v1: S x vote_chunks hit counts, then offsets
   int64_t hits_cap;
   int64_t *count;  // S located counts
   int32_t *const *o_freq;   // S pointers or null
@@ -258,6 +260,13 @@ struct V12Launch {
   // (see k_bucket_fft_rows); estimation reads them contiguously
   bool rows_layout;
 };
+// Blocks per signal of the v1 vote: its candidates (the ranges of the
+// selected buckets of loops 0..Lloc-thr) in chunks of 8 * 256.
+inline int vote_chunks(int n, int B_loc, int num, int Lloc, int thr)
+{
+  const int64_t total = (int64_t)std::max(Lloc - thr + 1, 0) * num * (n / B_loc);
+  return (int)std::max<int64_t>(1, (total + 2047) / 2048);
+}
 // Largest comb transform / selection array kept in LDS.
 constexpr int kMaxLdsComb = 8192;
 constexpr int kMaxLdsSelect = 16384;

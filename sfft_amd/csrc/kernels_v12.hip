@@ -6,8 +6,8 @@
 //   fold_fft      permuted-filter bucketization folded into B buckets, then
 //                 the B-point FFT in LDS (cf12 222-275)
 //   select        top-2k buckets of each location loop (cf12 278-302)
-//   vote_v1       exact-integer location votes (cf12 91-116) without an
-//                 n-sized score array
+//   vote          v1 exact-integer location votes (cf12 91-116) without an
+//                 n-sized score array (count / scan / write over chunks)
 //   estimate      per located frequency, the lower median over loops of
 //                 bucket / filter (cf12 340-419)
 //
@@ -428,68 +428,126 @@ __global__ void __launch_bounds__(kBlock) k_select(V12Launch a)
   block_top_num(mag, B, a.num, a.J + ((int64_t)s * a.Lloc + l) * a.num, sh);
 }
 
-// One block per signal.  A frequency f is a hit when, in at least thr of the
-// location loops, its permuted position p = f*ai mod n lies in the range
+// A frequency f is a hit when, in at least thr of the location loops, its
+// permuted position p = f*ai mod n lies in the range
 // [ceil((J-1/2) n/B), ceil((J+1/2) n/B)) of a selected bucket J
-// (src/computefourier-1.0-2.0.cc:91-116).  Candidates come from the ranges
-// of loops 0..Lloc-thr; a candidate is counted only from the first such loop
-// that selects it.  LDS: Lloc * B bits of selected-bucket flags.
-__global__ void __launch_bounds__(kBlock) k_vote_v1(V12Launch a)
+// (src/computefourier-1.0-2.0.cc:91-116).  Candidates are enumerated from the
+// ranges of loops 0..Lloc-thr (candidate c: loop c / (num*m), then selected
+// bucket, then offset in the range); a candidate is counted only from the
+// first such loop that selects it, so every hit appears once.  The candidate
+// list of a signal is cut into chunks of kVoteChunk, one block each:
+// k_vote_count counts each chunk's hits, k_vote_scan turns the counts into
+// offsets, k_vote_write writes the hits in candidate order.  LDS: Lloc * B
+// bits of selected-bucket flags.
+constexpr int kVoteChunk = 8 * kBlock;
+static_assert(kVoteChunk == 2048, "vote_chunks() assumes chunks of 2048 candidates");
+
+struct VoteCtx {
+  const int *prm, *J;
+  const unsigned *selbits;
+  int64_t total, per_loop;
+  unsigned mask;
+  int words, half, logm;
+};
+
+__device__ VoteCtx vote_setup(const V12Launch &a, int s, unsigned *selbits)
+{
+  VoteCtx v;
+  const int B = a.B_loc, Lloc = a.Lloc, num = a.num, n = a.n;
+  v.words = (B + 31) / 32;
+  v.prm = a.params + (int64_t)s * (2 * a.L + max(a.comb_loops, 1));
+  v.mask = (unsigned)n - 1;
+  const int m = n / B;
+  v.half = m / 2;
+  v.logm = ilog2(m);
+  v.per_loop = (int64_t)num * m;
+  v.total = (int64_t)max(Lloc - a.thr + 1, 0) * v.per_loop;
+  v.J = a.J + (int64_t)s * Lloc * num;
+  v.selbits = selbits;
+  for (int i = threadIdx.x; i < Lloc * v.words; i += blockDim.x) selbits[i] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < Lloc * num; i += blockDim.x) {
+    const int l = i / num, b = v.J[i];
+    atomicOr(&selbits[l * v.words + (b >> 5)], 1u << (b & 31));
+  }
+  __syncthreads();
+  return v;
+}
+
+__device__ __forceinline__ int vote_keep(const V12Launch &a, const VoteCtx &v, int64_t c,
+                                         unsigned &loc)
+{
+  if (c >= v.total) return 0;
+  const int l0 = (int)(c / v.per_loop);
+  const int64_t r = c % v.per_loop;
+  const int jb = v.J[(int64_t)l0 * a.num + (int)(r >> v.logm)];
+  const int q = (int)(r & ((1 << v.logm) - 1));
+  const unsigned m = 1u << v.logm;
+  const unsigned p = ((unsigned)jb * m - (unsigned)v.half + (unsigned)q + (unsigned)a.n) & v.mask;
+  loc = (p * (unsigned)v.prm[l0]) & v.mask;  // original frequency
+  int score = 0, earlier = 0;
+  for (int l = 0; l < a.Lloc; l++) {
+    const unsigned pl = (loc * (unsigned)v.prm[a.L + l]) & v.mask;
+    const unsigned bk = ((pl + (unsigned)v.half) & v.mask) >> v.logm;
+    const int sel = (v.selbits[l * v.words + (bk >> 5)] >> (bk & 31)) & 1;
+    score += sel;
+    earlier |= (l < l0) & sel;
+  }
+  return (score >= a.thr) && !earlier;
+}
+
+// grid: S x chunks
+__global__ void __launch_bounds__(kBlock) k_vote_count(V12Launch a, int chunks)
 {
   extern __shared__ unsigned selbits[];
   __shared__ int sh[16];
-  __shared__ int sh_base;
-  const int s = blockIdx.x;
-  const int B = a.B_loc, Lloc = a.Lloc, num = a.num, n = a.n;
-  const int words = (B + 31) / 32;
-  const int stride = 2 * a.L + max(a.comb_loops, 1);
-  const int *prm = a.params + (int64_t)s * stride;
-  const unsigned mask = (unsigned)n - 1;
-  const int m = n / B, half = m / 2, logm = ilog2(m);
-  for (int i = threadIdx.x; i < Lloc * words; i += blockDim.x) selbits[i] = 0;
-  __syncthreads();
-  const int *J = a.J + (int64_t)s * Lloc * num;
-  for (int i = threadIdx.x; i < Lloc * num; i += blockDim.x) {
-    const int l = i / num, b = J[i];
-    atomicOr(&selbits[l * words + (b >> 5)], 1u << (b & 31));
+  const int s = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+  const VoteCtx v = vote_setup(a, s, selbits);
+  int mine = 0;
+  for (int i = 0; i < kVoteChunk; i += blockDim.x) {
+    unsigned loc;
+    mine += vote_keep(a, v, (int64_t)ch * kVoteChunk + i + threadIdx.x, loc);
   }
-  if (threadIdx.x == 0) sh_base = 0;
-  __syncthreads();
+  int tot;
+  (void)block_exclusive_scan(mine, sh, tot);
+  if (threadIdx.x == 0) a.vote_cnt[blockIdx.x] = tot;
+}
+
+// grid: S; exclusive scan of a signal's chunk counts in place, and its
+// located count (at most hits_cap)
+__global__ void __launch_bounds__(kBlock) k_vote_scan(V12Launch a, int chunks)
+{
+  __shared__ int sh[16];
+  int *cnt = a.vote_cnt + (int64_t)blockIdx.x * chunks;
+  int64_t base = 0;
+  for (int c0 = 0; c0 < chunks; c0 += blockDim.x) {
+    const int c = c0 + threadIdx.x;
+    const int v = c < chunks ? cnt[c] : 0;
+    int tot;
+    const int pos = block_exclusive_scan(v, sh, tot);
+    if (c < chunks) cnt[c] = (int)min(base + pos, (int64_t)INT_MAX);
+    base += tot;
+  }
+  if (threadIdx.x == 0) a.count[blockIdx.x] = min(base, a.hits_cap);
+}
+
+// grid: S x chunks
+__global__ void __launch_bounds__(kBlock) k_vote_write(V12Launch a, int chunks)
+{
+  extern __shared__ unsigned selbits[];
+  __shared__ int sh[16];
+  const int s = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+  const VoteCtx v = vote_setup(a, s, selbits);
   int *hits = a.hits + (int64_t)s * a.hits_cap;
-  const int enum_loops = Lloc - a.thr + 1;
-  const int64_t per_loop = (int64_t)num * m;
-  const int64_t total = (int64_t)enum_loops * per_loop;
-  for (int64_t c0 = 0; c0 < total; c0 += blockDim.x) {
-    const int64_t c = c0 + threadIdx.x;
-    int keep = 0;
+  int64_t base = a.vote_cnt[blockIdx.x];
+  for (int i = 0; i < kVoteChunk; i += blockDim.x) {
     unsigned loc = 0;
-    if (c < total) {
-      const int l0 = (int)(c / per_loop);
-      const int64_t r = c % per_loop;
-      const int jb = J[(int64_t)l0 * num + (int)(r >> logm)];
-      const int q = (int)(r & (m - 1));
-      const unsigned p = ((unsigned)jb * (unsigned)m - (unsigned)half +
-                          (unsigned)q + (unsigned)n) & mask;
-      loc = (p * (unsigned)prm[l0]) & mask;  // original frequency
-      int score = 0, earlier = 0;
-      for (int l = 0; l < Lloc; l++) {
-        const unsigned pl = (loc * (unsigned)prm[a.L + l]) & mask;
-        const unsigned bk = ((pl + (unsigned)half) & mask) >> logm;
-        const int sel = (selbits[l * words + (bk >> 5)] >> (bk & 31)) & 1;
-        score += sel;
-        earlier |= (l < l0) & sel;
-      }
-      keep = (score >= a.thr) && !earlier;
-    }
+    const int keep = vote_keep(a, v, (int64_t)ch * kVoteChunk + i + threadIdx.x, loc);
     int tot;
     const int pos = block_exclusive_scan(keep, sh, tot);
-    const int base = sh_base;
     if (keep && base + pos < a.hits_cap) hits[base + pos] = (int)loc;
-    __syncthreads();
-    if (threadIdx.x == 0) sh_base = base + tot;
-    __syncthreads();
+    base += tot;
   }
-  if (threadIdx.x == 0) a.count[s] = min((int64_t)sh_base, a.hits_cap);
 }
 
 // ---------------------------------------------------------------- estimate
@@ -1171,8 +1229,14 @@ int select_vote_v1(const V12Launch &a, hipStream_t st)
   }
   if (launched("select") != 0) return -1;
   const size_t lb = (size_t)a.Lloc * ((a.B_loc + 31) / 32) * 4;
-  hipLaunchKernelGGL(k_vote_v1, dim3(a.S), dim3(kBlock), lb, st, a);
-  return launched("vote_v1");
+  const int chunks = vote_chunks(a.n, a.B_loc, a.num, a.Lloc, a.thr);
+  const dim3 grid((unsigned)((int64_t)a.S * chunks));
+  hipLaunchKernelGGL(k_vote_count, grid, dim3(kBlock), lb, st, a, chunks);
+  if (launched("vote_count") != 0) return -1;
+  hipLaunchKernelGGL(k_vote_scan, dim3(a.S), dim3(kBlock), 0, st, a, chunks);
+  if (launched("vote_scan") != 0) return -1;
+  hipLaunchKernelGGL(k_vote_write, grid, dim3(kBlock), lb, st, a, chunks);
+  return launched("vote_write");
 }
 
 template <int LMAX>

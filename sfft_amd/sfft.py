@@ -222,25 +222,22 @@ class sfft:
         import torch
         self._bind_stream(torch)
         num = len(xs)
+        rows = np.arange(num, dtype=np.uint64)
         if isinstance(xs, (list, tuple)):
-            ptrs = [t.data_ptr() for t in xs]
+            ins = np.array([t.data_ptr() for t in xs], dtype=np.uint64)
         else:
-            xb, xst = xs.data_ptr(), xs.stride(0) * xs.element_size()
-            ptrs = [xb + i * xst for i in range(num)]
-        ins = (C.c_void_p * num)(*ptrs)
-        outs = (_lib.SfftOutput * num)()
-        cap = freq.shape[1]
-        # row pointers by arithmetic (indexing a tensor per row costs microseconds)
-        fb, fs = freq.data_ptr(), freq.stride(0) * freq.element_size()
-        vb, vs = val.data_ptr(), val.stride(0) * val.element_size()
-        for i in range(num):
-            outs[i].capacity = cap
-            outs[i].freq = fb + i * fs
-            outs[i].val = vb + i * vs
-        r = self._L.sfft_exec_many_sparse(self.sfft_plan, num, ins, outs)
+            ins = np.uint64(xs.data_ptr()) + rows * np.uint64(xs.stride(0) * xs.element_size())
+        # sfft_output_c records {count, capacity, freq, val} built as one
+        # array: per-row ctypes field writes cost about a microsecond each
+        outs = np.zeros((num, 4), dtype=np.uint64)
+        outs[:, 1] = freq.shape[1]
+        outs[:, 2] = np.uint64(freq.data_ptr()) + rows * np.uint64(freq.stride(0) * freq.element_size())
+        outs[:, 3] = np.uint64(val.data_ptr()) + rows * np.uint64(val.stride(0) * val.element_size())
+        r = self._L.sfft_exec_many_sparse(self.sfft_plan, num, ins.ctypes.data_as(C.c_void_p),
+                                          outs.ctypes.data_as(C.POINTER(_lib.SfftOutput)))
         if r != 0:
             raise SfftError(_lib.last_error())
-        return np.array([outs[i].count for i in range(num)], dtype=np.int64)
+        return outs[:, 0].astype(np.int64)
 
     def filter(self, which):
         """The plan's filter `which` (0 location / Gauss-1, 1 estimation /
@@ -310,14 +307,17 @@ def execute_many_plans(plans, xs, outputs=None):
         outputs = [(torch.empty(p.capacity, dtype=torch.int32, device=x.device),
                     torch.empty(p.capacity, dtype=torch.complex128, device=x.device))
                    for p, x in zip(plans, xs)]
-    pl = (C.c_void_p * num)(*[C.cast(p.sfft_plan, C.c_void_p).value for p in plans])
-    ins = (C.c_void_p * num)(*[x.data_ptr() for x in xs])
-    outs = (_lib.SfftOutput * num)()
-    for i, (fr, va) in enumerate(outputs):
-        outs[i].capacity = int(fr.numel())
-        outs[i].freq = fr.data_ptr()
-        outs[i].val = va.data_ptr()
-    if _lib.load().sfft_exec_many_plans(num, pl, ins, outs) != 0:
+    pl = np.array([C.cast(p.sfft_plan, C.c_void_p).value for p in plans], dtype=np.uint64)
+    ins = np.array([x.data_ptr() for x in xs], dtype=np.uint64)
+    # sfft_output_c records {count, capacity, freq, val}
+    outs = np.zeros((num, 4), dtype=np.uint64)
+    outs[:, 1] = [fr.numel() for fr, _ in outputs]
+    outs[:, 2] = [fr.data_ptr() for fr, _ in outputs]
+    outs[:, 3] = [va.data_ptr() for _, va in outputs]
+    if _lib.load().sfft_exec_many_plans(num, pl.ctypes.data_as(C.c_void_p),
+                                        ins.ctypes.data_as(C.c_void_p),
+                                        outs.ctypes.data_as(C.POINTER(_lib.SfftOutput))) != 0:
         raise SfftError(_lib.last_error())
-    return [(fr[:min(outs[i].count, fr.numel())], va[:min(outs[i].count, va.numel())])
+    counts = outs[:, 0].astype(np.int64)
+    return [(fr[:min(int(counts[i]), fr.numel())], va[:min(int(counts[i]), va.numel())])
             for i, (fr, va) in enumerate(outputs)]

@@ -147,18 +147,19 @@ def test_fp32_refine_matches_fp64(torch_cuda, version, n, k, snr):
     assert err.max() <= 1e-6, err.max()
 
 
-def test_batch_equals_single(torch_cuda):
+@pytest.mark.parametrize("version", [1, 2])
+def test_batch_equals_single(torch_cuda, version):
     """exec_many of a batch gives each signal the same result as alone,
     when the RNG draws are replayed in signal order."""
     torch = torch_cuda
     n, k = 1 << 18, 64
     xs = [signal_np(n, k, seed=s, snr_db=30.0)[0] for s in range(6)]
-    plan = _gpu_plan(2, n, k)
+    plan = _gpu_plan(version, n, k)
     xd = [torch.from_numpy(x).cuda() for x in xs]
     fr, va = plan.make_sparse_outputs(len(xs))
     counts = plan.execute_many_sparse(xd, fr, va)
     torch.cuda.synchronize()
-    single = _gpu_plan(2, n, k)
+    single = _gpu_plan(version, n, k)
     for i, x in enumerate(xs):
         f1, v1 = single.execute_sparse_host(x)
         c = counts[i]
@@ -232,3 +233,23 @@ def test_graph_replays_match_eager(torch_cuda, version, n, k, snr, precision):
             c = int(ce[i])
             assert torch.equal(fg[i, :c], fe[i, :c])
             assert torch.equal(vg[i, :c], ve[i, :c])
+
+
+@pytest.mark.parametrize("version,n,k,snr", [(1, 1 << 18, 100, 30.0), (2, 1 << 20, 64, 30.0)])
+def test_stage_timing_keeps_results(torch_cuda, version, n, k, snr):
+    """Per-stage HIP-event timing: the stages that ran report time, and the
+    timed execs give the untimed results."""
+    print(f"case timing v{version} n={n} k={k}", flush=True)
+    x, f, v = signal_np(n, k, seed=4100 + k, snr_db=snr)
+    plain = _gpu_plan(version, n, k)
+    timed = _gpu_plan(version, n, k)
+    timed.enable_timing(True)
+    for _ in range(3):
+        a = plain.execute_sparse_host(x)
+        b = timed.execute_sparse_host(x)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    st = timed.stage_times()
+    timed.enable_timing(False)
+    ran = ["fold", "estimate"] + (["select_vote"] if version == 1 else ["comb"])
+    for name in ran:
+        assert st[name][1] > 0 and st[name][0] > 0, (name, st)
