@@ -9,8 +9,11 @@
  * caller needs (precision, seeded RNG streams, streams, errors).
  *
  * Pointers passed to sfft_exec / sfft_exec_many may be host or device
- * (HIP) pointers; the library detects which.  Device-resident inputs avoid
- * the PCIe copy of the whole signal.
+ * (HIP) pointers; the library detects which on every call (one
+ * hipPointerGetAttributes per allocation met in the call; nothing is
+ * remembered between calls, because a freed device range can come back as
+ * host memory).  Device-resident inputs avoid the PCIe copy of the whole
+ * signal.
  */
 #ifndef SFFT_AMD_SFFT_H
 #define SFFT_AMD_SFFT_H

@@ -28,47 +28,45 @@ Plan *get(const sfft_plan *plan)
 // Device allocations seen recently: a batch's signals usually live in one
 // or a few allocations, so one runtime query serves them all (a query costs
 // microseconds; a batch has hundreds of pointers).
-struct DeviceRanges {
-  std::mutex mu;
-  uintptr_t lo[16] = {}, hi[16] = {};
-  int used = 0, next = 0;
-  bool find(uintptr_t a)
+// Device-pointer test with a cache of the allocation ranges met during ONE
+// API call (a batch of rows of one tensor costs one query).  The cache must
+// not outlive the call: once the caller frees an allocation, its address
+// range can be handed out again as host memory, and a stale entry would
+// make the kernels dereference a host address.
+class PtrKinds {
+ public:
+  bool is_device(const void *p)
   {
-    std::lock_guard<std::mutex> lk(mu);
-    for (int i = 0; i < used; i++)
-      if (lo[i] <= a && a < hi[i]) return true;
-    return false;
-  }
-  void add(uintptr_t b, size_t size)
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    lo[next] = b;
-    hi[next] = b + size;
-    next = (next + 1) % 16;
-    used = std::min(used + 1, 16);
-  }
-} g_ranges;
-
-bool is_device(const void *p)
-{
-  if (!p) return false;
-  if (g_ranges.find(reinterpret_cast<uintptr_t>(p))) return true;
-  hipPointerAttribute_t attr;
-  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  const bool dev = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
-  if (dev) {
-    hipDeviceptr_t base = nullptr;
-    size_t size = 0;
-    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && size)
-      g_ranges.add(reinterpret_cast<uintptr_t>(base), size);
-    else
+    if (!p) return false;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    for (int i = 0; i < used_; i++)
+      if (lo_[i] <= a && a < hi_[i]) return true;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
       (void)hipGetLastError();
+      return false;
+    }
+    const bool dev = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+    if (dev) {
+      hipDeviceptr_t base = nullptr;
+      size_t size = 0;
+      if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && size) {
+        lo_[next_] = reinterpret_cast<uintptr_t>(base);
+        hi_[next_] = lo_[next_] + size;
+        next_ = (next_ + 1) % kSlots;
+        used_ = std::min(used_ + 1, kSlots);
+      } else {
+        (void)hipGetLastError();
+      }
+    }
+    return dev;
   }
-  return dev;
-}
+
+ private:
+  static constexpr int kSlots = 8;
+  uintptr_t lo_[kSlots] = {}, hi_[kSlots] = {};
+  int used_ = 0, next_ = 0;
+};
 
 // Signals per GPU batch: bounded by workspace and, for host inputs, by the
 // staging buffer (whole signals are copied to the device).
@@ -88,13 +86,14 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
   const int64_t n = P->n;
   const int64_t cap = P->output_capacity();
   bool host_in = false;
+  PtrKinds kinds;
   std::vector<char> in_dev(num);
   for (int s = 0; s < num; s++) {
     if (!in[s]) {
       set_error("null input signal");
       return -1;
     }
-    in_dev[s] = is_device(in[s]);
+    in_dev[s] = kinds.is_device(in[s]);
     host_in |= !in_dev[s];
   }
   const int chunk = chunk_signals(P, host_in);
@@ -133,7 +132,7 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
           set_error("null output signal");
           return -1;
         }
-        if (is_device(o)) {
+        if (kinds.is_device(o)) {
           de[s] = reinterpret_cast<double2 *>(o);
           if (k::zero_dense(de[s], n, st) != 0) return -1;
         } else {
@@ -142,8 +141,8 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
         }
       } else {
         sfft_output_c &o = sparse[c0 + s];
-        if (o.freq && o.val && o.capacity >= cap && is_device(o.freq) &&
-            is_device(o.val)) {
+        if (o.freq && o.val && o.capacity >= cap && kinds.is_device(o.freq) &&
+            kinds.is_device(o.val)) {
           fr[s] = o.freq;
           va[s] = reinterpret_cast<double2 *>(o.val);
         } else {
@@ -311,11 +310,12 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
   // group the signals by plan, in order of first appearance
   std::vector<Plan *> order;
   std::vector<std::vector<int>> members;
+  PtrKinds kinds;
   for (int i = 0; i < num; i++) {
     Plan *P = get(plans[i]);
     if (!P) return -1;
-    if (!in[i] || !is_device(in[i]) || !out[i].freq || !out[i].val || !is_device(out[i].freq) ||
-        !is_device(out[i].val)) {
+    if (!in[i] || !kinds.is_device(in[i]) || !out[i].freq || !out[i].val ||
+        !kinds.is_device(out[i].freq) || !kinds.is_device(out[i].val)) {
       set_error("sfft_exec_many_plans takes device inputs and device sparse outputs");
       return -1;
     }

@@ -942,33 +942,35 @@ __global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0,
   }
 }
 
-constexpr int kRI = 32;  // residues per tile of k_estimate_rows (8 per wavefront)
+// Residues per tile of k_estimate_rows (RI / 4 per wavefront): 32 for
+// batches, 4 when a batch gives too few tiles to fill the chip.
+constexpr int kRI = 32, kRIsmall = 4;
 
 // FULL: L == LMAX, so the loop over loops has no branch and the compiler
 // issues all bucket loads of a frequency before the first wait.
-template <int LMAX, bool FULL, bool FP32>
+template <int LMAX, bool FULL, bool FP32, int RI>
 __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles_i, int tiles_j)
 {
-  __shared__ int t_rb[kRI * LMAX];       // r * rows of hashed0
-  __shared__ int t_q[kRI * LMAX];        // q0 * u^-1 mod rows
-  __shared__ double2 t_g[kRI * LMAX];    // FP32 form: G / |G|^2; FP64: G
-  __shared__ double t_den[FP32 ? 1 : kRI * LMAX];  // FP64: |G|^2
+  __shared__ int t_rb[RI * LMAX];       // r * rows of hashed0
+  __shared__ int t_q[RI * LMAX];        // q0 * u^-1 mod rows
+  __shared__ double2 t_g[RI * LMAX];    // FP32 form: G / |G|^2; FP64: G
+  __shared__ double t_den[FP32 ? 1 : RI * LMAX];  // FP64: |G|^2
   __shared__ int s_off[LMAX];
-  __shared__ double2 stage[64 * (kRI + 1)];  // [row][residue], padded
+  __shared__ double2 stage[64 * (RI + 1)];  // [row][residue], padded
   int s;
   int64_t c;
   if (!xcd_block(blockIdx.x, a.S, (int64_t)tiles_i * tiles_j, s, c)) return;
   const int ti = (int)(c % tiles_i), tj = (int)(c / tiles_i);
   const int nc = a.num_comb[s];
   const int rows = a.n / a.W;
-  if (ti * kRI >= nc) return;
+  if (ti * RI >= nc) return;
   const int L = a.L;
   const int stride = 2 * L + max(a.comb_loops, 1);
   const int *approved = a.approved + (int64_t)s * a.comb_loops * a.num;
   if (threadIdx.x < L) s_off[threadIdx.x] = (int)loop_offset(threadIdx.x, a.Lloc, a.B_loc, a.B_est);
-  for (int e = threadIdx.x; e < kRI * L; e += blockDim.x) {
-    const int l = e / kRI, il = e % kRI;
-    const int i = ti * kRI + il;
+  for (int e = threadIdx.x; e < RI * L; e += blockDim.x) {
+    const int l = e / RI, il = e % RI;
+    const int i = ti * RI + il;
     if (i < nc) {
       const unsigned ai = (unsigned)a.params[(int64_t)s * stride + L + l];
       int hashed;
@@ -978,13 +980,13 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
       const int d = B / rows;
       const unsigned uinv = inv_odd(ai) & (unsigned)(rows - 1);
       const double den = G.x * G.x + G.y * G.y;
-      t_rb[l * kRI + il] = (hashed % d) * rows;
-      t_q[l * kRI + il] = (int)(((unsigned)(hashed / d) * uinv) & (unsigned)(rows - 1));
+      t_rb[l * RI + il] = (hashed % d) * rows;
+      t_q[l * RI + il] = (int)(((unsigned)(hashed / d) * uinv) & (unsigned)(rows - 1));
       if (FP32) {
-        t_g[l * kRI + il] = make_double2(G.x / den, G.y / den);
+        t_g[l * RI + il] = make_double2(G.x / den, G.y / den);
       } else {
-        t_g[l * kRI + il] = G;
-        t_den[l * kRI + il] = den;
+        t_g[l * RI + il] = G;
+        t_den[l * RI + il] = den;
       }
     }
   }
@@ -995,16 +997,16 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int j = tj * 64 + lane;             // one alias row per lane
   const int jm = j & (rows - 1);
-  for (int q = 0; q < kRI / 4; q++) {
+  for (int q = 0; q < RI / 4; q++) {
     asm volatile("" ::: "memory");
-    const int il = wid * (kRI / 4) + q;     // one residue per wavefront
-    if (ti * kRI + il >= nc) break;         // wave-uniform; no barrier inside
+    const int il = wid * (RI / 4) + q;     // one residue per wavefront
+    if (ti * RI + il >= nc) break;         // wave-uniform; no barrier inside
     double vr[LMAX], vi[LMAX];
 #pragma unroll
     for (int l = 0; l < LMAX; l++) {
       if (FULL || l < L) {
-        const int pos = t_rb[l * kRI + il] + ((t_q[l * kRI + il] + jm) & (rows - 1));
-        const double2 g = t_g[l * kRI + il];
+        const int pos = t_rb[l * RI + il] + ((t_q[l * RI + il] + jm) & (rows - 1));
+        const double2 g = t_g[l * RI + il];
         if (FP32) {
           const float2 u = xs32[s_off[l] + pos];
           const double ux = (double)u.x, uy = (double)u.y;
@@ -1013,23 +1015,23 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
           vi[l] = conj ? qq : -qq;
         } else {
           // the reference's operation order (cf12 383-398), as k_estimate_comb
-          divide_by_filter(xs64[s_off[l] + pos], g, t_den[l * kRI + il], conj, vr[l], vi[l]);
+          divide_by_filter(xs64[s_off[l] + pos], g, t_den[l * RI + il], conj, vr[l], vi[l]);
         }
       } else {
         vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
         vi[l] = vr[l];
       }
     }
-    stage[lane * (kRI + 1) + il] = lower_median<LMAX>(vr, vi, FULL ? LMAX : L);
+    stage[lane * (RI + 1) + il] = lower_median<LMAX>(vr, vi, FULL ? LMAX : L);
   }
   __syncthreads();
-  // row by row: kRI consecutive output slots per row
-  for (int e = threadIdx.x; e < 64 * kRI; e += blockDim.x) {
-    const int jr = e / kRI, il = e % kRI;
-    const int i = ti * kRI + il, jj = tj * 64 + jr;
+  // row by row: RI consecutive output slots per row
+  for (int e = threadIdx.x; e < 64 * RI; e += blockDim.x) {
+    const int jr = e / RI, il = e % RI;
+    const int i = ti * RI + il, jj = tj * 64 + jr;
     if (i < nc && jj < rows)
       write_result(a, s, (int64_t)jj * nc + i, (unsigned)jj * (unsigned)a.W + (unsigned)approved[i],
-                   stage[jr * (kRI + 1) + il]);
+                   stage[jr * (RI + 1) + il]);
   }
 }
 
@@ -1239,28 +1241,39 @@ int select_vote_v1(const V12Launch &a, hipStream_t st)
   return launched("vote_write");
 }
 
+// Blocks that keep every CU busy with several wavefronts.
+constexpr int64_t kFillBlocks = 1024;
+
+template <int LMAX, int RI>
+void launch_rows(const V12Launch &a, hipStream_t st, int tiles_j)
+{
+  const int tiles_i = (a.comb_loops * a.num + RI - 1) / RI;
+  const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));
+  if (a.fp32) {
+    if (a.L == LMAX)
+      hipLaunchKernelGGL((k_estimate_rows<LMAX, true, true, RI>), grid, dim3(kBlock), 0, st, a,
+                         tiles_i, tiles_j);
+    else
+      hipLaunchKernelGGL((k_estimate_rows<LMAX, false, true, RI>), grid, dim3(kBlock), 0, st, a,
+                         tiles_i, tiles_j);
+  } else {
+    if (a.L == LMAX)
+      hipLaunchKernelGGL((k_estimate_rows<LMAX, true, false, RI>), grid, dim3(kBlock), 0, st, a,
+                         tiles_i, tiles_j);
+    else
+      hipLaunchKernelGGL((k_estimate_rows<LMAX, false, false, RI>), grid, dim3(kBlock), 0, st,
+                         a, tiles_i, tiles_j);
+  }
+}
+
 template <int LMAX>
 void launch_estimate(const V12Launch &a, hipStream_t st)
 {
   if (a.comb_loops > 0 && a.rows_layout) {
-    const int tiles_i = (a.comb_loops * a.num + kRI - 1) / kRI;
     const int tiles_j = (a.n / a.W + 63) / 64;
-    const dim3 grid((unsigned)xcd_grid(a.S, (int64_t)tiles_i * tiles_j));
-    if (a.fp32) {
-      if (a.L == LMAX)
-        hipLaunchKernelGGL((k_estimate_rows<LMAX, true, true>), grid, dim3(kBlock), 0, st, a,
-                           tiles_i, tiles_j);
-      else
-        hipLaunchKernelGGL((k_estimate_rows<LMAX, false, true>), grid, dim3(kBlock), 0, st, a,
-                           tiles_i, tiles_j);
-    } else {
-      if (a.L == LMAX)
-        hipLaunchKernelGGL((k_estimate_rows<LMAX, true, false>), grid, dim3(kBlock), 0, st, a,
-                           tiles_i, tiles_j);
-      else
-        hipLaunchKernelGGL((k_estimate_rows<LMAX, false, false>), grid, dim3(kBlock), 0, st, a,
-                           tiles_i, tiles_j);
-    }
+    const int64_t big = (int64_t)a.S * ((a.comb_loops * a.num + kRI - 1) / kRI) * tiles_j;
+    if (big >= kFillBlocks) launch_rows<LMAX, kRI>(a, st, tiles_j);
+    else launch_rows<LMAX, kRIsmall>(a, st, tiles_j);
   } else if (a.comb_loops > 0) {
     const int tiles_i = (a.comb_loops * a.num + kTI - 1) / kTI;
     const int tiles_j = (a.n / a.W + kTJ - 1) / kTJ;

@@ -113,3 +113,30 @@ def test_reference_harness_input(torch_cuda, version):
     sup = np.nonzero(X)[0]
     assert np.all(np.isin(sup, np.nonzero(out)[0]))
     assert np.abs(out[sup] - 1.0).max() < 0.1   # src/verification.cc:37-58
+
+
+def test_host_input_after_freed_device_memory(torch_cuda):
+    """Host signals after device buffers the library has already seen were
+    freed: the freed address ranges can come back as host memory, so the
+    library must not remember them as device memory across calls."""
+    torch = torch_cuda
+    n, k = 1 << 20, 50
+    warm = sfft_amd.sfft(n, k, 1, strict=False, seed=(17, 4242))
+    for rep in range(3):
+        print(f"case freed device memory, round {rep}", flush=True)
+        xd = torch.zeros((8, n), dtype=torch.complex128, device="cuda")  # 128 MiB
+        fr, va = warm.make_sparse_outputs(8)
+        warm.execute_many_sparse(xd, fr, va)
+        torch.cuda.synchronize()
+        del xd, fr, va
+        torch.cuda.empty_cache()
+        for i in range(2):
+            x = signal_np(n, k, seed=90 + 2 * rep + i)[0]
+            on_host = sfft_amd.sfft(n, k, 1, strict=False, seed=(17, 4242))
+            on_dev = sfft_amd.sfft(n, k, 1, strict=False, seed=(17, 4242))
+            fh, vh = on_host.execute_sparse_host(x)
+            f1, v1 = on_dev.make_sparse_outputs(1)
+            c = on_dev.execute_many_sparse([torch.from_numpy(x).cuda()], f1, v1)
+            torch.cuda.synchronize()
+            assert np.array_equal(fh, f1[0, :int(c[0])].cpu().numpy())
+            assert np.array_equal(vh, v1[0, :int(c[0])].cpu().numpy())

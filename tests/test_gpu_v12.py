@@ -253,3 +253,29 @@ def test_stage_timing_keeps_results(torch_cuda, version, n, k, snr):
     ran = ["fold", "estimate"] + (["select_vote"] if version == 1 else ["comb"])
     for name in ran:
         assert st[name][1] > 0 and st[name][0] > 0, (name, st)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_large_batch_tiles_equal_single(torch_cuda, precision):
+    """A batch large enough for the wide estimation tiles (16 signals of
+    n=2^22, k=64) gives every signal the bits it gets alone, where the
+    narrow tiles run."""
+    torch = torch_cuda
+    from sfft_amd.signals import signals_device
+    n, k, S = 1 << 22, 64, 16
+    print(f"case wide tiles n={n} k={k} S={S} {precision}", flush=True)
+    x, _, _ = signals_device(n, k, list(range(500, 500 + S)), snr_db=40.0)
+    batch = _gpu_plan(2, n, k, precision)
+    fr, va = batch.make_sparse_outputs(S)
+    counts = batch.execute_many_sparse(x, fr, va)
+    single = _gpu_plan(2, n, k, precision)
+    f1, v1 = single.make_sparse_outputs(1)
+    for i in range(S):
+        c1 = single.execute_many_sparse(x[i:i + 1], f1, v1)
+        torch.cuda.synchronize()
+        c = int(counts[i])
+        assert int(c1[0]) == c
+        assert torch.equal(fr[i, :c], f1[0, :c])
+        assert torch.equal(va[i, :c], v1[0, :c])
+    del x
+    torch.cuda.empty_cache()
