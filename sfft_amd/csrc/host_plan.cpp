@@ -725,7 +725,8 @@ int exec_batch(Plan *P, BatchIO &io)
     t_dense[s] = io.dense ? io.dense[s] : nullptr;
   }
 
-  if (!P->ws->params.ensure(param_bytes) || !P->ws->ptrs.ensure(4 * tab_bytes) ||
+  // params and the pointer tables share one device buffer (one upload)
+  if (!P->ws->params.ensure(param_bytes + 4 * tab_bytes) ||
       (need64 && !P->ws->xsamp.ensure((size_t)S * xs * 16)) ||
       (fp32 && !P->ws->xsamp32.ensure((size_t)S * xs * 8)) ||
       !P->ws->approved.ensure((size_t)S * std::max(cl, 1) * num * 4) ||
@@ -748,7 +749,7 @@ int exec_batch(Plan *P, BatchIO &io)
       (big32 && !P->ws->tmp32.ensure((size_t)S * xs * 8)))
     return -1;
   hipStream_t st = P->stream;
-  char *dtab = reinterpret_cast<char *>(P->ws->ptrs.p);
+  char *dtab = reinterpret_cast<char *>(P->ws->params.p) + param_bytes;
 
   k::V12Launch a{};
   a.n = n;
@@ -831,9 +832,7 @@ int exec_batch(Plan *P, BatchIO &io)
     auto mark = [&](int i) {
       if (timed) (void)hipEventRecord(P->ev[i], st);
     };
-    HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * stride * sizeof(int),
-                          hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
+    HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, param_bytes + 4 * tab_bytes,
                           hipMemcpyHostToDevice, st));
     if (p.comb) {
       hipStream_t st2 = P->stream2;

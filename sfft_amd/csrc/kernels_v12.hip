@@ -39,8 +39,10 @@ inline int64_t loop_offset_host(int l, int Lloc, int Bl, int Be)
 }
 
 // ---------------------------------------------------------------- comb
-// One block per (signal, comb loop).  LDS: W double2.
-__global__ void __launch_bounds__(kBlock) k_comb(V12Launch a)
+// One block per (signal, comb loop).  LDS: W double2.  TB threads: kBlock
+// for batches; more for a few signals, whose comb is on the latency path.
+template <int TB>
+__global__ void __launch_bounds__(TB) k_comb(V12Launch a)
 {
   extern __shared__ double2 smem64[];
   __shared__ int sh[256 + 16];
@@ -56,12 +58,12 @@ __global__ void __launch_bounds__(kBlock) k_comb(V12Launch a)
   __syncthreads();
   lds_fft(smem64, logW, a.tw64, a.tw_nmax / W);
   // |.|^2 into the same LDS, as doubles (staged through registers: W is at
-  // most kMaxLdsFft64 = 32 * kBlock)
-  constexpr int kPer = kMaxLdsFft64 / kBlock;
+  // most kMaxLdsComb)
+  constexpr int kPer = (kMaxLdsComb + TB - 1) / TB;
   double v[kPer];
 #pragma unroll
   for (int q = 0; q < kPer; q++) {
-    const int i = threadIdx.x + q * kBlock;
+    const int i = threadIdx.x + q * TB;
     if (i < W) {
       const double2 z = smem64[i];
       v[q] = z.x * z.x + z.y * z.y;
@@ -71,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_comb(V12Launch a)
   double *mag = reinterpret_cast<double *>(smem64);
 #pragma unroll
   for (int q = 0; q < kPer; q++) {
-    const int i = threadIdx.x + q * kBlock;
+    const int i = threadIdx.x + q * TB;
     if (i < W) mag[i] = v[q];
   }
   __syncthreads();
@@ -1109,12 +1111,21 @@ int launch_failed(const char *what)
 
 }  // namespace
 
+// Below this many comb transforms per batch they run with kCombWide threads.
+constexpr int kFewCombs = 64, kCombWide = 1024;
+
 int comb(const V12Launch &a, hipStream_t st)
 {
   if (a.W <= kMaxLdsComb) {
     const size_t lds = (size_t)a.W * sizeof(double2);
-    allow_lds(k_comb, lds);
-    hipLaunchKernelGGL(k_comb, dim3(a.S * a.comb_loops), dim3(kBlock), lds, st, a);
+    if (a.S * a.comb_loops < kFewCombs) {
+      allow_lds(k_comb<kCombWide>, lds);
+      hipLaunchKernelGGL(k_comb<kCombWide>, dim3(a.S * a.comb_loops), dim3(kCombWide), lds, st,
+                         a);
+    } else {
+      allow_lds(k_comb<kBlock>, lds);
+      hipLaunchKernelGGL(k_comb<kBlock>, dim3(a.S * a.comb_loops), dim3(kBlock), lds, st, a);
+    }
     if (launched("comb") != 0) return -1;
   } else {
     const int64_t total = (int64_t)a.S * a.comb_loops * a.W;
