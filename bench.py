@@ -192,11 +192,15 @@ def main():
     fold_avg_ms = fold_ms / max(1, fold_launches)
     achieved = fold_bytes / (fold_avg_ms * 1e-3) / 1e9
     kname = "k_fold_fft32" if schedule32[0] == 0 else "k_fold_stream<true>"
+    traffic = measured_traffic(kname, S, n, k)
     result["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": measured_traffic(kname, S, n, k),
+                          "traffic": traffic,
                           "alg_bytes_per_launch": fold_bytes,
                           "avg_launch_ms": fold_avg_ms, "launches": fold_launches}
+    if traffic:  # line traffic the counters show, over this run's launch time
+        result["roofline"]["traffic_gbs"] = traffic / (fold_avg_ms * 1e-3) / 1e9
+        result["roofline"]["traffic_frac"] = result["roofline"]["traffic_gbs"] / HBM_PEAK_GBS
     result["roofline"]["copy_gbs"] = copy_rate(torch, dev)
     result["stages_ms_per_step"] = {s: v[0] / args.steps for s, v in stages32.items() if v[1]}
 
