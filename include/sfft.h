@@ -194,6 +194,15 @@ int sfft_plan_params(const sfft_plan *plan, int *p10);
  * bad version. */
 int sfft_derive_params(int n, int k, sfft_version version, int *p10);
 
+/* v3 outcome flags of the signals of the plan's last exec call, in signal
+ * order (for sfft_exec_many_plans: this plan's signals in call order): bit 0
+ * the reference's early return (src/computefourier-3.0.cc:859-860, empty
+ * result), bit 1 result map overflow (the call failed), bit 2 the
+ * estimate/subtract loop stopped at its 3*4096-pass cap instead of at
+ * repeated bucket counts.  Copies up to num flags; returns the number of
+ * signals recorded (0 for v1/v2 plans), or -1 for a bad plan. */
+int sfft_plan_last_status(const sfft_plan *plan, int *flags, int num);
+
 /* Stage timing with HIP events recorded on the plan's stream around each
  * kernel stage of every later exec (0 disables, any other value enables and
  * resets the totals). */

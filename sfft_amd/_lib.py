@@ -18,6 +18,7 @@ EXPORTS = [
     "sfft_plan_memory", "sfft_last_error", "sfft_plan_params",
     "sfft_derive_params", "sfft_design_filter", "sfft_debug_buckets",
     "sfft_plan_enable_timing", "sfft_plan_stage_times", "sfft_plan_filter",
+    "sfft_plan_last_status",
     "sfft_plan_set_schedule", "sfft_plan_get_schedule", "sfft_plan_autotune",
     "sfft_exec_many_plans", "sfft_plan_enable_graphs", "sfft_generate_input",
 ]
@@ -100,6 +101,7 @@ def load():
     L.sfft_plan_enable_timing.argtypes = [PP, C.c_int]
     L.sfft_plan_filter.argtypes = [PP, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_int)]
     L.sfft_plan_stage_times.argtypes = [PP, P, P]
+    L.sfft_plan_last_status.argtypes = [PP, P, C.c_int]
     L.sfft_plan_enable_graphs.argtypes = [PP, C.c_int]
     L.sfft_generate_input.argtypes = [C.c_int, C.c_int, C.c_long, C.c_double, P, P,
                                       C.POINTER(C.c_double)]

@@ -82,6 +82,7 @@ int chunk_signals(const Plan *P, bool host_inputs)
 int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
         sfft_output_c *sparse)
 {
+  P->last_status.clear();
   if (num <= 0) return 0;
   const int64_t n = P->n;
   const int64_t cap = P->output_capacity();
@@ -332,6 +333,7 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
     members[g].push_back(i);
   }
   const size_t G = order.size();
+  for (Plan *P : order) P->last_status.clear();
   std::vector<std::vector<const double2 *>> ins(G);
   std::vector<std::vector<int32_t *>> frs(G);
   std::vector<std::vector<double2 *>> vas(G);
@@ -520,6 +522,15 @@ int sfft_plan_enable_timing(sfft_plan *plan, int enabled)
     P->stage_launches[i] = 0;
   }
   return 0;
+}
+
+int sfft_plan_last_status(const sfft_plan *plan, int *flags, int num)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  const int m = (int)P->last_status.size();
+  for (int i = 0; i < std::min(m, std::max(num, 0)) && flags; i++) flags[i] = P->last_status[i];
+  return m;
 }
 
 int sfft_plan_stage_times(const sfft_plan *plan, double *ms6, int64_t *launches6)

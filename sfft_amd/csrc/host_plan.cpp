@@ -265,6 +265,11 @@ static int make_v3_tables(Plan *P)
   P->v3_lds = std::max<size_t>(q.W_Man <= kMaxLdsFft64 ? (size_t)q.W_Man * 16 : 0,
                                (size_t)(2 * q.B_g1 + 2 * q.B_g2) * 16);
   P->v3_lds = std::max<size_t>(P->v3_lds, std::min<size_t>((size_t)P->v3_Mcap * 8, 131072));
+  // room past the Gauss bucket arrays for the Gauss-2 phase chain, where it
+  // fits
+  const size_t gauss = (size_t)(2 * q.B_g1 + 2 * q.B_g2) * 16;
+  P->v3_lds = std::max<size_t>(P->v3_lds,
+                               std::min<size_t>(gauss + (size_t)(q.w_g2 + 1) * 16, 160 * 1024));
   if (P->v3_lds > 160 * 1024) {
     set_error("v3 Gauss filters above 2560 buckets are not supported");
     return -1;
@@ -592,7 +597,7 @@ static int v3_collect(const V3Pending &pd, int S, hipStream_t st)
 static int v3_finish(Plan *P, BatchIO &io, const V3Pending &pd)
 {
   const int S = io.S;
-  P->last_status.assign(pd.h_status, pd.h_status + S);
+  P->last_status.insert(P->last_status.end(), pd.h_status, pd.h_status + S);
   if (io.counts)
     for (int s = 0; s < S; s++) io.counts[s] = pd.h_counts[s];
   for (int s = 0; s < S; s++)

@@ -143,7 +143,7 @@ struct Plan {
   int v3_half1 = 0, v3_half2 = 0;
   int v3_Mcap = 0, v3_Hcap = 0, v3_Fcap = 0, v3_Rcap = 0;
   size_t v3_lds = 0;
-  std::vector<int> last_status;  // per signal of the last v3 batch
+  std::vector<int> last_status;  // v3: per signal of the last exec call
 
   Workspace own_ws;
   Workspace *ws = &own_ws;  // a clone points at its server worker's

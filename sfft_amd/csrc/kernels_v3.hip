@@ -532,7 +532,13 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
     // permute_fourier (cf3 66-87) forms the phase of sample i as
     // shift * step^i by repeated complex multiplication; one lane replays
     // that chain so the samples carry the reference's rounding.
-    double2 *phase = a.phase + (int64_t)s * a.Pcap;
+    // The chain goes to LDS past the four Gauss bucket arrays when it fits
+    // (a lone lane's global stores made it several times slower), else to
+    // the signal's global scratch.
+    const size_t ph_off = 2 * (size_t)a.B1 + 2 * (size_t)a.B2;
+    double2 *phase = (ph_off + a.w2 + 1) * sizeof(double2) <= a.lds_bytes
+                         ? lds + ph_off
+                         : a.phase + (int64_t)s * a.Pcap;
     if (threadIdx.x == 0) {
       double s0, c0, s1, c1;
       sincos(2 * M_PI * c.gG * c.b / n, &s0, &c0);
@@ -674,9 +680,11 @@ __global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a) { v3_signal(a, bloc
 
 // Signals of several plans (different n, k) in one launch: block b runs
 // signal blocks[b].y of plan group blocks[b].x.
-__global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *groups,
-                                                       const int2 *blocks)
+__global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *__restrict__ groups,
+                                                       const int2 *__restrict__ blocks)
 {
+  // restrict: the descriptors are read-only here, so their fields need not
+  // be reloaded after every store inside the signal's loops
   const int2 b = blocks[blockIdx.x];
   v3_signal(groups[b.x], b.y);
 }

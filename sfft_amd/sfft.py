@@ -256,6 +256,16 @@ class sfft:
         full[d % self.length] = r
         return t, full
 
+    def last_status(self):
+        """v3 flags per signal of the last exec call: 1 early return,
+        2 map overflow, 4 loop stopped at the pass cap."""
+        m = self._L.sfft_plan_last_status(self.sfft_plan, None, 0)
+        if m < 0:
+            raise SfftError(_lib.last_error())
+        flags = (C.c_int * max(m, 1))()
+        self._L.sfft_plan_last_status(self.sfft_plan, flags, m)
+        return [flags[i] for i in range(m)]
+
     STAGES = ["comb", "fold", "bucket_fft", "select_vote", "estimate", "v3"]
 
     def enable_timing(self, enabled=True):

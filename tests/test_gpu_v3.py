@@ -117,3 +117,22 @@ def test_v3_uneven_k_one_launch(torch_cuda):
             assert int(res[i][0].numel()) == c
             assert torch.equal(res[i][0], fr[j, :c])
             assert torch.equal(res[i][1], va[j, :c])
+
+
+def test_v3_status_flags(torch_cuda):
+    """sfft_plan_last_status: one flag word per signal of the last call;
+    bit 0 matches the oracle's early return (cf3 859-860)."""
+    n, k = 1 << 16, 50
+    for seed in (5000 + k, 11, 12):
+        x, f, v = signal_np(n, k, seed=seed)
+        plan = sfft_amd.sfft(n, k, 3, seed=(17, 4242))
+        ref = _oracle_v3(n, k, x, _plan_filters(plan))
+        plan.execute_sparse_host(x)
+        st = plan.last_status()
+        print(f"case status seed={seed} flags={st} early={ref['early']}", flush=True)
+        assert len(st) == 1
+        assert bool(st[0] & 1) == bool(ref["early"])
+        assert st[0] & 2 == 0
+    v2 = sfft_amd.sfft(n, k, 2, strict=False, seed=(17, 4242))
+    v2.execute_sparse_host(x)
+    assert v2.last_status() == []
