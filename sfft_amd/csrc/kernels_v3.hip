@@ -23,6 +23,7 @@ using namespace sfa::dev;
 namespace {
 
 constexpr int kV3Block = 512;
+static_assert((kV3Block & (kV3Block - 1)) == 0, "owner tests take residues by masking");
 // Subtraction passes of up to this many pairs run on one thread per bucket
 // array; longer ones on bucket-owner threads.
 constexpr int kSerialApply = 128;
@@ -160,8 +161,13 @@ __device__ void gauss_apply_pair(double2 *S0, double2 *S1, int B, int h, const d
 __device__ void gauss_apply_owned(double2 *S0, double2 *S1, int B, const int *remi, int slot,
                                   const double2 *rem, int roff, int cnt)
 {
-  const int T = blockDim.x, t = threadIdx.x;
+  constexpr int T = kV3Block;
+  const int t = threadIdx.x;
   if (t >= B) return;
+  // B and T are powers of two: the buckets h-1, h, h+1 of a pair can only
+  // include one of this thread's (b mod T == t) when (h - t + 1) mod
+  // min(B, T) <= 2; only those pairs take the exact test below
+  const int M = min(B, T);
   for (int i0 = 0; i0 < cnt; i0 += 8) {
     int hh[8];
 #pragma unroll
@@ -170,6 +176,7 @@ __device__ void gauss_apply_owned(double2 *S0, double2 *S1, int B, const int *re
     for (int u = 0; u < 8; u++) {
       const int h = hh[u];
       if (h < 0) break;
+      if ((unsigned)((h - t + 1) & (M - 1)) > 2u) continue;
       const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
 #pragma unroll
       for (int q = 0; q < 3; q++) {
@@ -241,7 +248,8 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
   if (which & 1) gauss_apply_owned(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt);
   if (which & 2) gauss_apply_owned(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt);
   if (which & 4) {
-    const int T = blockDim.x, t = threadIdx.x;
+    constexpr int T = kV3Block;
+    const int t = threadIdx.x;
     for (int i0 = 0; i0 < cnt; i0 += 8) {
       int hh[8];
 #pragma unroll
@@ -250,7 +258,7 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
       for (int u = 0; u < 8; u++) {
         const int h = hh[u];
         if (h < 0) break;
-        if (h % T != t) continue;
+        if ((h & (T - 1)) != t) continue;  // h = key mod W >= 0
         const double2 *r = c.rem + (int64_t)(i0 + u) * 14 + 12;
         c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
         c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
