@@ -27,6 +27,8 @@ static_assert((kV3Block & (kV3Block - 1)) == 0, "owner tests take residues by ma
 // Subtraction passes of up to this many pairs run on one thread per bucket
 // array; longer ones on bucket-owner threads.
 constexpr int kSerialApply = 128;
+// Bucket indices an owner thread loads ahead in the subtraction scans.
+constexpr int kApplyAhead = 32;
 
 __device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
 
@@ -157,9 +159,11 @@ __device__ void gauss_apply_pair(double2 *S0, double2 *S1, int B, int h, const d
 // Pair i subtracts r[0..2] from buckets h, h+1, h-1 of S0 and r[3..5] from
 // those of S1 (in that order; with B <= 2 a bucket can be several of them).
 // Thread t owns the buckets b = t mod blockDim and takes the pairs in pair
-// order, 8 bucket indices loaded ahead.
-__device__ void gauss_apply_owned(double2 *S0, double2 *S1, int B, const int *remi, int slot,
-                                  const double2 *rem, int roff, int cnt)
+// order, kApplyAhead bucket indices loaded ahead (each batch costs one
+// memory latency).
+__device__ void gauss_apply_owned(double2 *__restrict__ S0, double2 *__restrict__ S1, int B,
+                                  const int *__restrict__ remi, int slot,
+                                  const double2 *__restrict__ rem, int roff, int cnt)
 {
   constexpr int T = kV3Block;
   const int t = threadIdx.x;
@@ -168,12 +172,13 @@ __device__ void gauss_apply_owned(double2 *S0, double2 *S1, int B, const int *re
   // include one of this thread's (b mod T == t) when (h - t + 1) mod
   // min(B, T) <= 2; only those pairs take the exact test below
   const int M = min(B, T);
-  for (int i0 = 0; i0 < cnt; i0 += 8) {
-    int hh[8];
+  for (int i0 = 0; i0 < cnt; i0 += kApplyAhead) {
+    int hh[kApplyAhead];
 #pragma unroll
-    for (int u = 0; u < 8; u++) hh[u] = i0 + u < cnt ? remi[(i0 + u) * 3 + slot] : -1;
+    for (int u = 0; u < kApplyAhead; u++)
+      hh[u] = i0 + u < cnt ? remi[(i0 + u) * 3 + slot] : -1;
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
+    for (int u = 0; u < kApplyAhead; u++) {
       const int h = hh[u];
       if (h < 0) break;
       if ((unsigned)((h - t + 1) & (M - 1)) > 2u) continue;
@@ -250,18 +255,22 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
   if (which & 4) {
     constexpr int T = kV3Block;
     const int t = threadIdx.x;
-    for (int i0 = 0; i0 < cnt; i0 += 8) {
-      int hh[8];
+    const int *__restrict__ remi = c.remi;
+    const double2 *__restrict__ rem = c.rem;
+    double2 *__restrict__ ms = c.MS;
+    for (int i0 = 0; i0 < cnt; i0 += kApplyAhead) {
+      int hh[kApplyAhead];
 #pragma unroll
-      for (int u = 0; u < 8; u++) hh[u] = i0 + u < cnt ? c.remi[(i0 + u) * 3 + 2] : -1;
+      for (int u = 0; u < kApplyAhead; u++)
+        hh[u] = i0 + u < cnt ? remi[(i0 + u) * 3 + 2] : -1;
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < kApplyAhead; u++) {
         const int h = hh[u];
         if (h < 0) break;
         if ((h & (T - 1)) != t) continue;  // h = key mod W >= 0
-        const double2 *r = c.rem + (int64_t)(i0 + u) * 14 + 12;
-        c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
-        c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
+        const double2 *r = rem + (int64_t)(i0 + u) * 14 + 12;
+        ms[2 * h] = csub(ms[2 * h], r[0]);
+        ms[2 * h + 1] = csub(ms[2 * h + 1], r[1]);
       }
     }
   }
