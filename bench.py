@@ -114,7 +114,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(batches shard by signal across GPUs with no exchange; see DESIGN.md)"
+              "(batches shard by signal across GPUs with no exchange)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     for v in ("RANK", "WORLD_SIZE"):
@@ -287,7 +287,8 @@ def main():
 
         # ------------------------------------------------------ accuracy
         # FP64 full FFT of the signal as the oracle; the reference's v1/v2
-        # values are conj(X), so compare with conj.
+        # values are conj(X) (src/computefourier-1.0-2.0.cc:383-398), so
+        # compare with conj.
         l1_loc, l1_true, off_energy = [], [], []
         for i in range(min(4, S)):
             Xf = torch.fft.fft(xs[i]) / n
