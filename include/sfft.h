@@ -14,6 +14,10 @@
  * remembered between calls, because a freed device range can come back as
  * host memory).  Device-resident inputs avoid the PCIe copy of the whole
  * signal.
+ *
+ * Threads: calls on one plan from several threads take turns (the plan's
+ * workspace, RNG streams and staging buffers are shared); different plans
+ * run concurrently, each on its own stream.
  */
 #ifndef SFFT_AMD_SFFT_H
 #define SFFT_AMD_SFFT_H

@@ -25,9 +25,10 @@ Plan *get(const sfft_plan *plan)
   return reinterpret_cast<Plan *>(plan->data);
 }
 
-// Device allocations seen recently: a batch's signals usually live in one
-// or a few allocations, so one runtime query serves them all (a query costs
-// microseconds; a batch has hundreds of pointers).
+// Calls on one plan from several threads take turns (the reference let
+// OpenMP threads call sfft_exec on one plan concurrently).
+std::unique_lock<std::mutex> hold(Plan *P) { return std::unique_lock<std::mutex>(P->mu); }
+
 // Device-pointer test with a cache of the allocation ranges met during ONE
 // API call (a batch of rows of one tensor costs one query).  The cache must
 // not outlive the call: once the caller frees an allocation, its address
@@ -262,6 +263,7 @@ void sfft_exec(sfft_plan *plan, complex_t *in, complex_t *out)
   set_error("");
   Plan *P = get(plan);
   if (!P) return;
+  auto lk = hold(P);
   const complex_t *ins[1] = {in};
   complex_t *outs[1] = {out};
   (void)run(P, 1, ins, outs, nullptr);
@@ -272,6 +274,7 @@ void sfft_exec_many(sfft_plan *plan, int num, complex_t **in, complex_t **out)
   set_error("");
   Plan *P = get(plan);
   if (!P) return;
+  auto lk = hold(P);
   (void)run(P, num, in, out, nullptr);
 }
 
@@ -286,6 +289,7 @@ int sfft_exec_sparse(sfft_plan *plan, const complex_t *in, sfft_output_c *out)
   set_error("");
   Plan *P = get(plan);
   if (!P || !out) return -1;
+  auto lk = hold(P);
   const complex_t *ins[1] = {in};
   return run(P, 1, ins, nullptr, out);
 }
@@ -296,6 +300,7 @@ int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
   set_error("");
   Plan *P = get(plan);
   if (!P || !out) return -1;
+  auto lk = hold(P);
   return run(P, num, in, nullptr, out);
 }
 
@@ -333,6 +338,10 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
     members[g].push_back(i);
   }
   const size_t G = order.size();
+  std::vector<Plan *> by_addr(order);
+  std::sort(by_addr.begin(), by_addr.end());
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (Plan *P : by_addr) locks.push_back(hold(P));  // address order: no deadlock
   for (Plan *P : order) P->last_status.clear();
   std::vector<std::vector<const double2 *>> ins(G);
   std::vector<std::vector<int32_t *>> frs(G);
@@ -375,6 +384,7 @@ int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   if (fold_mode < 0 || fold_mode > 2 || stream_blocks < 0 || stream_blocks > (1 << 20)) {
     set_error("fold_mode must be 0, 1 or 2 and stream_blocks in [0, 2^20]");
     return -1;
@@ -388,6 +398,7 @@ int sfft_plan_get_schedule(const sfft_plan *plan, int *fold_mode, int *stream_bl
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   if (fold_mode) *fold_mode = P->fold_mode;
   if (stream_blocks) *stream_blocks = P->stream_blocks;
   return 0;
@@ -397,6 +408,7 @@ int sfft_plan_enable_graphs(sfft_plan *plan, int enabled)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   P->use_graphs = enabled != 0;
   return 0;
 }
@@ -406,6 +418,7 @@ int sfft_plan_autotune(sfft_plan *plan, int batch)
   set_error("");
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   return autotune_plan(P, batch);
 }
 
@@ -413,6 +426,7 @@ int sfft_plan_set_precision(sfft_plan *plan, sfft_precision precision)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   if (precision != SFFT_PRECISION_FP64 && precision != SFFT_PRECISION_FP32_REFINE) {
     set_error("unknown precision");
     return -1;
@@ -425,6 +439,7 @@ int sfft_plan_seed(sfft_plan *plan, unsigned int random_seed, long drand48_seed)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   P->rng.seed(random_seed, drand48_seed);
   return 0;
 }
@@ -433,6 +448,7 @@ int sfft_plan_set_stream(sfft_plan *plan, void *hip_stream)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   (void)hipStreamSynchronize(P->stream);
   if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
   P->own_stream = false;
@@ -451,6 +467,7 @@ int sfft_plan_set_reference_conjugate(sfft_plan *plan, int enabled)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   P->ref_conj = enabled != 0;
   return 0;
 }
@@ -460,6 +477,7 @@ int sfft_plan_memory(const sfft_plan *plan, int64_t *plan_bytes,
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   if (plan_bytes) *plan_bytes = P->plan_bytes();
   if (bytes_per_signal) *bytes_per_signal = P->bytes_per_signal();
   return 0;
@@ -516,6 +534,7 @@ int sfft_plan_enable_timing(sfft_plan *plan, int enabled)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   P->timing = enabled != 0;
   for (int i = 0; i < Plan::kStages; i++) {
     P->stage_ms[i] = 0;
@@ -528,6 +547,7 @@ int sfft_plan_last_status(const sfft_plan *plan, int *flags, int num)
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   const int m = (int)P->last_status.size();
   for (int i = 0; i < std::min(m, std::max(num, 0)) && flags; i++) flags[i] = P->last_status[i];
   return m;
@@ -537,6 +557,7 @@ int sfft_plan_stage_times(const sfft_plan *plan, double *ms6, int64_t *launches6
 {
   Plan *P = get(plan);
   if (!P) return -1;
+  auto lk = hold(P);
   for (int i = 0; i < Plan::kStages; i++) {
     if (ms6) ms6[i] = P->stage_ms[i];
     if (launches6) launches6[i] = P->stage_launches[i];
@@ -549,6 +570,7 @@ int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
 {
   Plan *P = get(plan);
   if (!P || !dst || P->version == SFFT_VERSION_3 || signal < 0) return -1;
+  auto lk = hold(P);
   const int64_t xs = P->v12.x_samp_size();
   if (signal >= P->last_S) {
     set_error("no such signal in the last batch");

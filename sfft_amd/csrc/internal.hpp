@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <mutex>
 #include <cstdint>
 #include <cstdlib>
 #include <string>
@@ -111,6 +112,9 @@ struct Workspace {
 };
 
 struct Plan {
+  // API calls on one plan are serialised on this (workspace, RNG streams and
+  // pinned staging are per plan)
+  std::mutex mu;
   sfft_version version = SFFT_VERSION_1;
   int n = 0, k = 0;
   V12Params v12;

@@ -279,3 +279,28 @@ def test_large_batch_tiles_equal_single(torch_cuda, precision):
         assert torch.equal(va[i, :c], v1[0, :c])
     del x
     torch.cuda.empty_cache()
+
+
+def test_concurrent_calls_on_one_plan(torch_cuda):
+    """Two host threads calling one plan take turns: each call gets the
+    draws of one of the two sequential calls, and together they get both."""
+    import threading
+    torch = torch_cuda
+    n, k = 1 << 18, 64
+    x, f, v = signal_np(n, k, seed=2500, snr_db=30.0)
+    seq = _gpu_plan(2, n, k)
+    ref = [seq.execute_sparse_host(x) for _ in range(2)]
+    shared = _gpu_plan(2, n, k)
+    got = [None, None]
+
+    def call(i):
+        got[i] = shared.execute_sparse_host(x)
+    ths = [threading.Thread(target=call, args=(i,)) for i in range(2)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    torch.cuda.synchronize()
+
+    def same(a, b):
+        return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    match = [[same(g, r) for r in ref] for g in got]
+    assert (match[0][0] and match[1][1]) or (match[0][1] and match[1][0]), match
