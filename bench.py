@@ -542,6 +542,12 @@ def cpu_baseline(args, check=None):
     t0 = time.perf_counter()
     P.exec_many(xp, a, ai, off, op, -threads)
     dt = time.perf_counter() - t0
+    # 1-thread latency: sequential single-signal execs (first 20 draws)
+    lat1 = []
+    for i in range(min(20, num)):
+        ts = time.perf_counter()
+        P.exec_many(xp[i:i + 1], a[i:i + 1], ai[i:i + 1], off[i:i + 1], op[:1], -1)
+        lat1.append((time.perf_counter() - ts) * 1e3)
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -556,7 +562,9 @@ def cpu_baseline(args, check=None):
                       "reference semantics incl. the dense zero-filled output; "
                       "OpenMP over signals; %.1f s wall"
                       % (num, distinct, int(np.log2(n)), k, args.snr, dt),
-            "host_cpus": os.cpu_count(), "cpu_model": cpu_model}
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model,
+            "latency_1thread_ms": {"p50": pct(lat1, 50), "p99": pct(lat1, 99),
+                                   "samples": len(lat1)}}
     if check is not None:
         out["parity"] = oracle_check(check, n, k)
     return out
