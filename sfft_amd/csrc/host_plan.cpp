@@ -617,8 +617,17 @@ static int exec_batch_v3(Plan *P, BatchIO &io)
     for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
     P->ev_made = true;
   }
+  // through the multi-plan kernel with one plan: it reads the descriptor
+  // from device memory (a by-value kernel argument whose address the
+  // kernel takes is copied to scratch and reloaded inside its loops)
+  std::vector<int2> blocks(io.S);
+  for (int s = 0; s < io.S; s++) blocks[s] = make_int2(0, s);
+  if (!P->ws->multi.ensure(sizeof(k::V3Launch) + blocks.size() * sizeof(int2) + 256)) return -1;
+  auto *d_groups = P->ws->multi.as<k::V3Launch>();
+  auto *d_blocks = reinterpret_cast<int2 *>(
+      (reinterpret_cast<uintptr_t>(d_groups + 1) + 255) & ~uintptr_t(255));
   if (P->timing) (void)hipEventRecord(P->ev[0], st);
-  if (k::v3(pd.a, st) != 0) return -1;
+  if (k::v3_multi(&pd.a, 1, d_groups, d_blocks, blocks.data(), io.S, st) != 0) return -1;
   if (P->timing) (void)hipEventRecord(P->ev[1], st);
   if (v3_collect(pd, io.S, st) != 0) return -1;
   HIP_OK(hipStreamSynchronize(st));

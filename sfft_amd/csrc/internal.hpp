@@ -309,7 +309,6 @@ struct V3Launch {
   int64_t o_cap;
   size_t lds_bytes;
 };
-int v3(const V3Launch &a, hipStream_t st);
 // G groups in one launch; d_groups / d_blocks: device room for the
 // descriptors and the nblocks (group, signal) pairs.
 int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,

@@ -693,8 +693,6 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   }
 }
 
-__global__ void __launch_bounds__(kV3Block) k_v3(V3Launch a) { v3_signal(a, blockIdx.x); }
-
 // Signals of several plans (different n, k) in one launch: block b runs
 // signal blocks[b].y of plan group blocks[b].x.
 __global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *__restrict__ groups,
@@ -740,19 +738,6 @@ static void allow_v3_lds(const void *kernel, size_t bytes)
 {
   if (bytes > 65536)
     (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-int v3(const V3Launch &a, hipStream_t st)
-{
-  if (mansour_prepass(a, st) != 0) return -1;
-  allow_v3_lds(reinterpret_cast<const void *>(k_v3), a.lds_bytes);
-  hipLaunchKernelGGL(k_v3, dim3(a.S), dim3(kV3Block), a.lds_bytes, st, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    set_error(std::string("v3: ") + hipGetErrorString(e));
-    return -1;
-  }
-  return 0;
 }
 
 int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,
