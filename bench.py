@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="Poisson arrival rate of the serving run, requests/s")
+    ap.add_argument("--serve-rate-high", type=float, default=2000.0,
+                    help="a second FP32+refine serving run at this rate (0: none)")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=16)
     ap.add_argument("--only-configs", action="store_true",
@@ -420,7 +422,9 @@ def serving(args, torch, dev):
     (noiseless) 50/50; v2 draws the plan cannot run (W_Comb < 2k+1) are
     rejected and counted.  Inputs and plans are device-resident before the
     arrivals start; the same trace runs in the FP32+refine and the FP64
-    forms.  Latency = submit to the end of the request's batch."""
+    forms, and once more in FP32+refine compressed to a higher rate (the
+    server then batches and overlaps requests).  Latency = submit to the
+    end of the request's batch."""
     from concurrent.futures import ThreadPoolExecutor
 
     from sfft_amd import SfftError
@@ -463,7 +467,12 @@ def serving(args, torch, dev):
            "rejected_infeasible": sum(c is None for c in caps),
            "mix": "n uniform {2^18,2^20,2^22,2^24}, k=floor(10*200^U), v2 SNR 30 dB / v3 "
                   "noiseless 50/50"}
-    for prec in ("fp32_refine", "fp64"):
+    runs = [("fp32_refine", args.serve_rate), ("fp64", args.serve_rate)]
+    if args.serve_rate_high > 0:
+        runs.append(("fp32_refine", args.serve_rate_high))
+    for prec, rate in runs:
+        name = prec if rate == args.serve_rate else "%s_at_%g_per_s" % (prec, rate)
+        at = arrivals * (args.serve_rate / rate)
         with Server(workers=args.serve_workers, max_batch=args.serve_batch) as srv:
             t0 = time.perf_counter()
             with ThreadPoolExecutor(8) as ex:
@@ -472,7 +481,7 @@ def serving(args, torch, dev):
             tickets, rejected = [], 0
             start = time.perf_counter()
             for i, (n, k, v) in enumerate(reqs):
-                dt = arrivals[i] - (time.perf_counter() - start)
+                dt = at[i] - (time.perf_counter() - start)
                 if dt > 0:
                     time.sleep(dt)
                 try:
@@ -483,7 +492,7 @@ def serving(args, torch, dev):
             lat = [srv.wait(t)[2] for t in tickets]
             span = time.perf_counter() - start
             st = srv.stats()
-        out[prec] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        out[name] = {"rate_per_s": rate, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                      "mean_ms": float(np.mean(lat)), "completed": st["completed"],
                      "rejected": rejected, "failed": st["failed"],
                      "transforms_per_s": st["completed"] / span,
