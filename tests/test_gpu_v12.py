@@ -168,14 +168,36 @@ def test_batch_equals_single(torch_cuda, version):
         assert np.abs(va[i, :c].cpu().numpy() - v1).max() == 0.0
 
 
-def test_host_pointers_dense(torch_cuda):
-    """sfft_exec with host in/out (the reference's own calling convention)."""
+@pytest.mark.parametrize("version", [1, 2])
+def test_host_pointers_dense(torch_cuda, version):
+    """sfft_exec with host in/out (the reference's own calling convention):
+    the dense, zero-filled output equals the oracle's bit for bit."""
     n, k = 1 << 16, 50
     x, f, v = signal_np(n, k, seed=5)
-    plan = sfft_amd.sfft(n, k, 1, seed=(17, 4242))
+    plan = sfft_amd.sfft(n, k, version, seed=(17, 4242))
     out = plan.execute(x)
-    ref = _oracle_run(1, n, k, x, _plan_filters(plan))
+    ref = _oracle_run(version, n, k, x, _plan_filters(plan))
     assert np.array_equal(out, ref["out"])
+
+
+def test_host_pointers_dense_v3(torch_cuda):
+    """v3 through sfft_exec with host in/out: the oracle's located
+    frequencies carry its values, every other entry is zero."""
+    n, k = 1 << 18, 100
+    x, f, v = signal_np(n, k, seed=6)
+    plan = sfft_amd.sfft(n, k, 3, seed=(17, 4242))
+    t1, f1 = plan.filter(0)
+    t2, f2 = plan.filter(1)
+    oracle.seed_glibc(17, 4242)
+    P = oracle.V3(n, k)
+    P.set_filters(t1, f1, t2, f2)
+    ref = P.exec(x, P.draw())
+    out = plan.execute(x)
+    if ref["early"]:
+        assert not np.any(out)
+        return
+    assert np.array_equal(np.nonzero(out)[0], np.sort(ref["freqs"][ref["vals"] != 0]))
+    assert np.abs(out[ref["freqs"]] - ref["vals"]).max() <= 1e-9 * np.abs(ref["vals"]).max()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
