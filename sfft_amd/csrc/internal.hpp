@@ -287,7 +287,7 @@ struct V3Launch {
   // per-signal workspace, strided by signal
   double2 *ms;               // 2W (interleaved loops)
   double2 *ms_pre, *ms_tmp;  // W above the LDS size: S x 2 x W transforms
-                             // computed before k_v3 (else null)
+                             // computed before k_v3_multi (else null)
   int *map_keys;             // Mcap (insertion order)
   double2 *map_vals;         // Mcap
   int *map_hidx;             // Hcap hash slots -> entry

@@ -5,9 +5,10 @@
 //
 // Parallel within a signal: the filters (gathers, folds, FFTs in LDS), the
 // per-bucket estimation (atan2 / sincos), the result-map updates and the
-// subtraction values.  The subtractions themselves are applied in the
-// reference's order (found list order, one wavefront per bucket array), so
-// every bucket sees the same sequence of roundings as in the oracle.  The
+// subtraction values.  The subtractions themselves reach every bucket in the
+// reference's order (found-list order: short passes by one thread per bucket
+// array, longer ones by the thread owning the bucket), so every bucket sees
+// the same sequence of roundings as in the oracle.  The
 // result map is iterated in insertion order (the reference's
 // tr1::unordered_map order is unreproducible; see the oracle).
 #include <algorithm>
