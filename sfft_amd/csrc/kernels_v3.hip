@@ -30,6 +30,9 @@ static_assert((kV3Block & (kV3Block - 1)) == 0, "owner tests take residues by ma
 constexpr int kSerialApply = 128;
 // Bucket indices an owner thread loads ahead in the subtraction scans.
 constexpr int kApplyAhead = 32;
+// Pairs per LDS chunk of the Gauss subtractions (fewer fitting than
+// kMinStageChunk: indices and values read from memory instead).
+constexpr int kStageChunk = 256, kMinStageChunk = 32;
 
 __device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
 
@@ -195,6 +198,45 @@ __device__ void gauss_apply_owned(double2 *__restrict__ S0, double2 *__restrict_
   }
 }
 
+// As gauss_apply_owned, a chunk of pairs at a time through LDS: the block
+// copies the chunk's bucket indices and removal values next to the bucket
+// arrays (one memory latency per chunk), then the owners apply them in pair
+// order from LDS.  Called by every thread of the block (barriers inside).
+__device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict__ S1, int B,
+                                   const int *__restrict__ remi, int slot,
+                                   const double2 *__restrict__ rem, int roff, int cnt,
+                                   double2 *__restrict__ sv, int *__restrict__ sh_idx,
+                                   int chunk)
+{
+  constexpr int T = kV3Block;
+  const int t = threadIdx.x;
+  const int M = min(B, T);
+  for (int base = 0; base < cnt; base += chunk) {
+    const int m = min(chunk, cnt - base);
+    __syncthreads();  // the previous chunk is applied
+    for (int j = t; j < m; j += T) sh_idx[j] = remi[(int64_t)(base + j) * 3 + slot];
+    for (int j = t; j < 6 * m; j += T) {
+      const int u = j / 6, q = j - 6 * u;
+      sv[j] = rem[(int64_t)(base + u) * 14 + roff + q];
+    }
+    __syncthreads();
+    if (t < B) {
+      for (int u = 0; u < m; u++) {
+        const int h = sh_idx[u];
+        if ((unsigned)((h - t + 1) & (M - 1)) > 2u) continue;
+        const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          if (bk[q] % T != t) continue;
+          S0[bk[q]] = csub(S0[bk[q]], sv[6 * u + q]);
+          S1[bk[q]] = csub(S1[bk[q]], sv[6 * u + 3 + q]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // update_mansour_loops2 (cf3 300-351)
 __device__ void mansour_removal(const Ctx &c, int key, double2 value, int &h, double2 *r)
 {
@@ -212,8 +254,8 @@ __device__ void mansour_removal(const Ctx &c, int key, double2 value, int &h, do
 // Subtraction values of cnt (key, value) pairs, then their application in
 // pair order.  which: bit 0 Gauss-1, bit 1 Gauss-2, bit 2 Mansour.
 // Slots per pair in rem: [0..5] G1, [6..11] G2, [12..13] Mansour.
-__device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int cnt,
-                         int which)
+__device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const double2 *vals,
+                                         int cnt, int which)
 {
   const V3Launch &a = *c.a;
   for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
@@ -251,8 +293,22 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
     __syncthreads();
     return;
   }
-  if (which & 1) gauss_apply_owned(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt);
-  if (which & 2) gauss_apply_owned(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt);
+  {
+    // LDS past the four Gauss bucket arrays: 6 double2 + 1 int per pair
+    extern __shared__ double2 lds[];
+    const size_t used = (2 * (size_t)c.B1 + 2 * (size_t)c.B2) * sizeof(double2);
+    const size_t room = a.lds_bytes > used ? a.lds_bytes - used : 0;
+    const int chunk = (int)min((size_t)kStageChunk, room / (6 * sizeof(double2) + sizeof(int)));
+    double2 *sv = lds + 2 * c.B1 + 2 * c.B2;
+    int *sh_idx = reinterpret_cast<int *>(sv + 6 * (size_t)chunk);
+    if (chunk >= kMinStageChunk) {
+      if (which & 1) gauss_apply_staged(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt, sv, sh_idx, chunk);
+      if (which & 2) gauss_apply_staged(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt, sv, sh_idx, chunk);
+    } else {
+      if (which & 1) gauss_apply_owned(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt);
+      if (which & 2) gauss_apply_owned(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt);
+    }
+  }
   if (which & 4) {
     constexpr int T = kV3Block;
     const int t = threadIdx.x;
@@ -282,7 +338,7 @@ __device__ void subtract(const Ctx &c, const int *keys, const double2 *vals, int
 // ---------------------------------------------------------------- estimation
 // Candidates per bucket into cf/cv (cf = -1: none), then compaction in
 // bucket order into ef/ev.  Returns the found count (uniform).
-__device__ int compact(const Ctx &c, int nb, int *sh)
+__device__ __forceinline__ int compact(const Ctx &c, int nb, int *sh)
 {
   const int T = blockDim.x;
   const int C = (nb + T - 1) / T;
@@ -304,7 +360,7 @@ __device__ int compact(const Ctx &c, int nb, int *sh)
 }
 
 // estimate_freq_gauss_loops2 (cf3 484-640) on (S0, S1), B buckets.
-__device__ int est_gauss(const Ctx &c, const double2 *S0, const double2 *S1, int B, int a_,
+__device__ __forceinline__ int est_gauss(const Ctx &c, const double2 *S0, const double2 *S1, int B, int a_,
                          int b_, const double2 *win, int half, int *sh)
 {
   const int n = c.n;
@@ -354,7 +410,7 @@ __device__ int est_gauss(const Ctx &c, const double2 *S0, const double2 *S1, int
 }
 
 // estimate_freq_mansour_loops2 (cf3 642-776)
-__device__ int est_mansour(const Ctx &c, int *sh)
+__device__ __forceinline__ int est_mansour(const Ctx &c, int *sh)
 {
   const int n = c.n, W = c.W;
   const double PI2 = 2 * M_PI;
@@ -401,7 +457,7 @@ __device__ int est_mansour(const Ctx &c, int *sh)
 
 // Adds the F found (ef, ev) to the map (new keys appended in found order).
 // Returns false when the map would overflow.
-__device__ bool map_add_found(const Ctx &c, int F, int *sh, int *s_size)
+__device__ __forceinline__ bool map_add_found(const Ctx &c, int F, int *sh, int *s_size)
 {
   const int T = blockDim.x;
   const int C = (F + T - 1) / T;
