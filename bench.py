@@ -397,7 +397,7 @@ def other_configs(torch, dev):
              torch.empty(p.capacity, dtype=torch.complex128, device=dev)) for p in order]
     execute_many_plans(order, xs, outs)  # warm-up
     torch.cuda.synchronize()
-    reps = 3
+    reps = 10
     t0 = time.perf_counter()
     for _ in range(reps):
         res = execute_many_plans(order, xs, outs)

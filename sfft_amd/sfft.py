@@ -180,8 +180,9 @@ class sfft:
         self._L.sfft_plan_memory(self.sfft_plan, C.byref(a), C.byref(b))
         return a.value, b.value
 
-    def _bind_stream(self, torch):
-        s = torch.cuda.current_stream().cuda_stream
+    def _bind_stream(self, torch, s=None):
+        if s is None:
+            s = torch.cuda.current_stream().cuda_stream
         if s != self._stream:
             self._L.sfft_plan_set_stream(self.sfft_plan, C.c_void_p(s))
             self._stream = s
@@ -311,13 +312,16 @@ def execute_many_plans(plans, xs, outputs=None):
     import torch
     num = len(xs)
     assert len(plans) == num
-    for p in plans:
-        p._bind_stream(torch)
+    stream = torch.cuda.current_stream().cuda_stream
+    addr = {}
+    for p in {id(p): p for p in plans}.values():
+        p._bind_stream(torch, stream)
+        addr[id(p)] = C.cast(p.sfft_plan, C.c_void_p).value
     if outputs is None:
         outputs = [(torch.empty(p.capacity, dtype=torch.int32, device=x.device),
                     torch.empty(p.capacity, dtype=torch.complex128, device=x.device))
                    for p, x in zip(plans, xs)]
-    pl = np.array([C.cast(p.sfft_plan, C.c_void_p).value for p in plans], dtype=np.uint64)
+    pl = np.array([addr[id(p)] for p in plans], dtype=np.uint64)
     ins = np.array([x.data_ptr() for x in xs], dtype=np.uint64)
     # sfft_output_c records {count, capacity, freq, val}
     outs = np.zeros((num, 4), dtype=np.uint64)
