@@ -201,7 +201,11 @@ __device__ void gauss_apply_owned(double2 *__restrict__ S0, double2 *__restrict_
 // As gauss_apply_owned, a chunk of pairs at a time through LDS: the block
 // copies the chunk's bucket indices and removal values next to the bucket
 // arrays (one memory latency per chunk), then the owners apply them in pair
-// order from LDS.  Called by every thread of the block (barriers inside).
+// order from LDS.  With B <= blockDim each thread owns one bucket and keeps
+// it in registers over the whole pass (written back at the end), and a wave
+// skips, by one vote, the pairs that touch none of its buckets; the
+// subtractions a bucket sees and their order are unchanged.  Called by
+// every thread of the block (barriers inside).
 __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict__ S1, int B,
                                    const int *__restrict__ remi, int slot,
                                    const double2 *__restrict__ rem, int roff, int cnt,
@@ -211,6 +215,12 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
   constexpr int T = kV3Block;
   const int t = threadIdx.x;
   const int M = min(B, T);
+  const bool one = B <= T;  // one bucket per owner thread (B a power of two)
+  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+  if (one && t < B) {
+    a0 = S0[t];
+    a1 = S1[t];
+  }
   for (int base = 0; base < cnt; base += chunk) {
     const int m = min(chunk, cnt - base);
     __syncthreads();  // the previous chunk is applied
@@ -220,6 +230,31 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
       sv[j] = rem[(int64_t)(base + u) * 14 + roff + q];
     }
     __syncthreads();
+    if (one) {
+      if (t < B) {
+        const int bm = B - 1;
+        for (int u = 0; u < m; u++) {
+          const int h = sh_idx[u];
+          // buckets h, h+1, h-1 (cf3 465-482), in that order
+          const bool m0 = h == t, m1 = ((h + 1) & bm) == t, m2 = ((h + bm) & bm) == t;
+          if (!__any(m0 || m1 || m2)) continue;
+          const double2 *r = sv + 6 * u;
+          if (m0) {
+            a0 = csub(a0, r[0]);
+            a1 = csub(a1, r[3]);
+          }
+          if (m1) {
+            a0 = csub(a0, r[1]);
+            a1 = csub(a1, r[4]);
+          }
+          if (m2) {
+            a0 = csub(a0, r[2]);
+            a1 = csub(a1, r[5]);
+          }
+        }
+      }
+      continue;
+    }
     if (t < B) {
       for (int u = 0; u < m; u++) {
         const int h = sh_idx[u];
@@ -233,6 +268,10 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
         }
       }
     }
+  }
+  if (one && t < B) {
+    S0[t] = a0;
+    S1[t] = a1;
   }
   __syncthreads();
 }
