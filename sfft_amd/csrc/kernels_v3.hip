@@ -202,9 +202,10 @@ __device__ void gauss_apply_owned(double2 *__restrict__ S0, double2 *__restrict_
 // copies the chunk's bucket indices and removal values next to the bucket
 // arrays (one memory latency per chunk), then the owners apply them in pair
 // order from LDS.  With B <= blockDim each thread owns one bucket and keeps
-// it in registers over the whole pass (written back at the end), and a wave
-// skips, by one vote, the pairs that touch none of its buckets; the
-// subtractions a bucket sees and their order are unchanged.  Called by
+// it in registers over the whole pass (written back at the end); it marks
+// the pairs that touch its bucket among each 32 in a bit mask and applies
+// just those, so the threads of a wave work on different pairs at once.
+// The subtractions a bucket sees and their order are unchanged.  Called by
 // every thread of the block (barriers inside).
 __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict__ S1, int B,
                                    const int *__restrict__ remi, int slot,
@@ -233,23 +234,36 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
     if (one) {
       if (t < B) {
         const int bm = B - 1;
-        for (int u = 0; u < m; u++) {
-          const int h = sh_idx[u];
-          // buckets h, h+1, h-1 (cf3 465-482), in that order
-          const bool m0 = h == t, m1 = ((h + 1) & bm) == t, m2 = ((h + bm) & bm) == t;
-          if (!__any(m0 || m1 || m2)) continue;
-          const double2 *r = sv + 6 * u;
-          if (m0) {
-            a0 = csub(a0, r[0]);
-            a1 = csub(a1, r[3]);
+        for (int g0 = 0; g0 < m; g0 += 32) {
+          // this thread's pairs among the next 32, as a bit mask ...
+          const int gm = min(32, m - g0);
+          unsigned own = 0;
+#pragma unroll
+          for (int uu = 0; uu < 32; uu++) {
+            if (uu < gm) {
+              const int h = sh_idx[g0 + uu];
+              const bool hit = h == t || ((h + 1) & bm) == t || ((h + bm) & bm) == t;
+              own |= (unsigned)hit << uu;
+            }
           }
-          if (m1) {
-            a0 = csub(a0, r[1]);
-            a1 = csub(a1, r[4]);
-          }
-          if (m2) {
-            a0 = csub(a0, r[2]);
-            a1 = csub(a1, r[5]);
+          // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482)
+          while (own) {
+            const int u = g0 + __ffs(own) - 1;
+            own &= own - 1;
+            const int h = sh_idx[u];
+            const double2 *r = sv + 6 * u;
+            if (h == t) {
+              a0 = csub(a0, r[0]);
+              a1 = csub(a1, r[3]);
+            }
+            if (((h + 1) & bm) == t) {
+              a0 = csub(a0, r[1]);
+              a1 = csub(a1, r[4]);
+            }
+            if (((h + bm) & bm) == t) {
+              a0 = csub(a0, r[2]);
+              a1 = csub(a1, r[5]);
+            }
           }
         }
       }
