@@ -64,7 +64,7 @@ class PtrKinds {
   }
 
  private:
-  static constexpr int kSlots = 8;
+  static constexpr int kSlots = 128;  // a batch's inputs are often separate allocations
   uintptr_t lo_[kSlots] = {}, hi_[kSlots] = {};
   int used_ = 0, next_ = 0;
 };

@@ -304,11 +304,12 @@ class sfft:
         return fr[:m].copy(), va[:m].copy()
 
 
-def execute_many_plans(plans, xs, outputs=None):
+def execute_many_plans(plans, xs, outputs=None, trim=True):
     """sfft_exec_many_plans: signal i of xs (device tensors) runs plans[i];
     the v3 signals of all plans run as one launch.  outputs: list of
     (freq, val) device tensors (allocated at each plan's capacity when
-    None).  Returns [(freq[:count], val[:count])]."""
+    None).  Returns [(freq[:count], val[:count])], or with trim=False the
+    counts (int64 array) the call wrote, leaving the views to the caller."""
     import torch
     num = len(xs)
     assert len(plans) == num
@@ -333,5 +334,7 @@ def execute_many_plans(plans, xs, outputs=None):
                                         outs.ctypes.data_as(C.POINTER(_lib.SfftOutput))) != 0:
         raise SfftError(_lib.last_error())
     counts = outs[:, 0].astype(np.int64)
+    if not trim:
+        return counts
     return [(fr[:min(int(counts[i]), fr.numel())], va[:min(int(counts[i]), va.numel())])
             for i, (fr, va) in enumerate(outputs)]

@@ -136,3 +136,32 @@ def test_v3_status_flags(torch_cuda):
     v2 = sfft_amd.sfft(n, k, 2, strict=False, seed=(17, 4242))
     v2.execute_sparse_host(x)
     assert v2.last_status() == []
+
+
+def test_v3_many_plans_counts_only(torch_cuda):
+    """trim=False returns the counts the call wrote; the outputs hold the
+    same results as a trimmed call of identically seeded plans."""
+    torch = torch_cuda
+    from sfft_amd.sfft import execute_many_plans
+    keys = [(1 << 20, 200), (1 << 18, 100)]
+    order = [0, 1, 0]
+    xs = [torch.from_numpy(signal_np(keys[g][0], keys[g][1], seed=950 + i)[0]).cuda()
+          for i, g in enumerate(order)]
+
+    def plans():
+        return [sfft_amd.sfft(n, k, 3, seed=(31 + g, 77 + g)) for g, (n, k) in enumerate(keys)]
+
+    pa = plans()
+    res = execute_many_plans([pa[g] for g in order], xs)
+    pb = plans()
+    outs = [(torch.empty(pb[g].capacity, dtype=torch.int32, device="cuda"),
+             torch.empty(pb[g].capacity, dtype=torch.complex128, device="cuda")) for g in order]
+    counts = execute_many_plans([pb[g] for g in order], xs, outs, trim=False)
+    torch.cuda.synchronize()
+    assert counts.dtype == np.int64 and len(counts) == len(order)
+    for i in range(len(order)):
+        c = int(counts[i])
+        print(f"case v3 counts-only signal {i} count {c}", flush=True)
+        assert c == res[i][0].numel() and c > 0
+        assert torch.equal(outs[i][0][:c], res[i][0])
+        assert torch.equal(outs[i][1][:c], res[i][1])
