@@ -27,7 +27,7 @@ constexpr int kV3Block = 512;
 static_assert((kV3Block & (kV3Block - 1)) == 0, "owner tests take residues by masking");
 // Subtraction passes of up to this many pairs run on one thread per bucket
 // array; longer ones on bucket-owner threads.
-constexpr int kSerialApply = 128;
+constexpr int kSerialApply = 16;
 // Bucket indices an owner thread loads ahead in the subtraction scans.
 constexpr int kApplyAhead = 32;
 // Pairs per LDS chunk of the Gauss subtractions (fewer fitting than
