@@ -241,28 +241,30 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
 #pragma unroll
           for (int uu = 0; uu < 32; uu++) {
             if (uu < gm) {
+              // t is one of h, h+1, h-1 mod B (exact for every power of two)
               const int h = sh_idx[g0 + uu];
-              const bool hit = h == t || ((h + 1) & bm) == t || ((h + bm) & bm) == t;
-              own |= (unsigned)hit << uu;
+              own |= (unsigned)((unsigned)((t - h + 1) & bm) <= 2u) << uu;
             }
           }
-          // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482)
+          // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482);
+          // the pair's values are loaded with its index, not after it
           while (own) {
             const int u = g0 + __ffs(own) - 1;
             own &= own - 1;
-            const int h = sh_idx[u];
             const double2 *r = sv + 6 * u;
+            const int h = sh_idx[u];
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
             if (h == t) {
-              a0 = csub(a0, r[0]);
-              a1 = csub(a1, r[3]);
+              a0 = csub(a0, r0);
+              a1 = csub(a1, r3);
             }
             if (((h + 1) & bm) == t) {
-              a0 = csub(a0, r[1]);
-              a1 = csub(a1, r[4]);
+              a0 = csub(a0, r1);
+              a1 = csub(a1, r4);
             }
             if (((h + bm) & bm) == t) {
-              a0 = csub(a0, r[2]);
-              a1 = csub(a1, r[5]);
+              a0 = csub(a0, r2);
+              a1 = csub(a1, r5);
             }
           }
         }
