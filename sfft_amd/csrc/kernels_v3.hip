@@ -674,6 +674,8 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
       sincos(2 * M_PI * c.ai * c.b / n, &s1, &c1);
       double2 sh = make_double2(c0, s0);
       const double2 st = make_double2(c1, s1);
+      // unrolled: the loop control and stores stay off the product chain
+#pragma unroll 8
       for (int i = 0; i <= a.w2; i++) {
         phase[i] = sh;
         sh = make_double2(sh.x * st.x - sh.y * st.y, sh.x * st.y + sh.y * st.x);
