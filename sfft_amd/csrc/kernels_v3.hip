@@ -31,7 +31,8 @@ constexpr int kSerialApply = 16;
 // Bucket indices an owner thread loads ahead in the subtraction scans.
 constexpr int kApplyAhead = 32;
 // Pairs per LDS chunk of the Gauss subtractions (fewer fitting than
-// kMinStageChunk: indices and values read from memory instead).
+// kMinStageChunk: indices and values read from memory instead); a chunk is
+// a multiple of 32, the owners' scan width.
 constexpr int kStageChunk = 256, kMinStageChunk = 32;
 
 __device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
@@ -235,17 +236,18 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
       if (t < B) {
         const int bm = B - 1;
         for (int g0 = 0; g0 < m; g0 += 32) {
-          // this thread's pairs among the next 32, as a bit mask ...
+          // this thread's pairs among the next 32, as a bit mask ... (the
+          // chunk is a multiple of 32, so all 32 reads stay in sh_idx;
+          // entries past m are stale and masked off)
           const int gm = min(32, m - g0);
           unsigned own = 0;
 #pragma unroll
           for (int uu = 0; uu < 32; uu++) {
-            if (uu < gm) {
-              // t is one of h, h+1, h-1 mod B (exact for every power of two)
-              const int h = sh_idx[g0 + uu];
-              own |= (unsigned)((unsigned)((t - h + 1) & bm) <= 2u) << uu;
-            }
+            // t is one of h, h+1, h-1 mod B (exact for every power of two)
+            const int h = sh_idx[g0 + uu];
+            own |= (unsigned)((unsigned)((t - h + 1) & bm) <= 2u) << uu;
           }
+          if (gm < 32) own &= (1u << gm) - 1u;
           // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482);
           // the pair's values are loaded with its index, not after it
           while (own) {
@@ -353,7 +355,8 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
     extern __shared__ double2 lds[];
     const size_t used = (2 * (size_t)c.B1 + 2 * (size_t)c.B2) * sizeof(double2);
     const size_t room = a.lds_bytes > used ? a.lds_bytes - used : 0;
-    const int chunk = (int)min((size_t)kStageChunk, room / (6 * sizeof(double2) + sizeof(int)));
+    const int chunk =
+        (int)min((size_t)kStageChunk, room / (6 * sizeof(double2) + sizeof(int))) / 32 * 32;
     double2 *sv = lds + 2 * c.B1 + 2 * c.B2;
     int *sh_idx = reinterpret_cast<int *>(sv + 6 * (size_t)chunk);
     if (chunk >= kMinStageChunk) {
