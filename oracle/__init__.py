@@ -108,6 +108,8 @@ def lib():
         L.or_v3_exec.argtypes = [C.POINTER(V3Plan), P, P, P, P, P, P, C.c_long]
         L.or_v3_exec.restype = C.c_long
         L.or_v12_exec_many.argtypes = [C.POINTER(V12Plan), C.c_int, P, P, P, P, P, C.c_int]
+        L.fm_fold32.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_uint, C.c_int, P]
+        L.fm_fft32.argtypes = [P, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -299,3 +301,29 @@ def seed_glibc(random_seed=17, drand48_seed=4242):
     libc.srand48.argtypes = [C.c_long]
     libc.srandom(random_seed)
     libc.srand48(drand48_seed)
+
+
+def fp32_buckets(x, n, B, taps, ai, seg_rows, nmax):
+    """The FP32 form's buckets of one loop after its B-point FFT (fp32_model.c:
+    the library's FP32 summation order, not the reference's): x the signal,
+    taps the filter's w taps (rounded to FP32 here as the plan does), ai the
+    loop's gather multiplier, nmax the plan's twiddle table size."""
+    L = lib()
+    xr = np.ascontiguousarray(x, dtype=np.complex128)
+    t32 = np.ascontiguousarray(np.stack([np.asarray(taps).real, np.asarray(taps).imag], 1)
+                               .astype(np.float32))
+    out = np.zeros(2 * B, np.float32)
+    L.fm_fold32(xr.ctypes.data, n, B, len(taps), t32.ctypes.data, int(ai) & 0xffffffff,
+                seg_rows, out.ctypes.data)
+    L.fm_fft32(out.ctypes.data, B, nmax)
+    return out[0::2].astype(np.float64) + 1j * out[1::2].astype(np.float64)
+
+
+def v12_draws(n, loops, comb_loops, W_comb):
+    """One exec's RNG draws (a, ai, comb offsets) from the process-global
+    glibc streams, without building an oracle plan."""
+    a = np.zeros(loops, dtype=np.int32)
+    ai = np.zeros(loops, dtype=np.int32)
+    off = np.zeros(max(1, comb_loops), dtype=np.int32)
+    lib().or_v12_draw(n, loops, comb_loops, W_comb, _p(a), _p(ai), _p(off))
+    return a, ai, off

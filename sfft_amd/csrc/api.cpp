@@ -29,45 +29,6 @@ Plan *get(const sfft_plan *plan)
 // OpenMP threads call sfft_exec on one plan concurrently).
 std::unique_lock<std::mutex> hold(Plan *P) { return std::unique_lock<std::mutex>(P->mu); }
 
-// Device-pointer test with a cache of the allocation ranges met during ONE
-// API call (a batch of rows of one tensor costs one query).  The cache must
-// not outlive the call: once the caller frees an allocation, its address
-// range can be handed out again as host memory, and a stale entry would
-// make the kernels dereference a host address.
-class PtrKinds {
- public:
-  bool is_device(const void *p)
-  {
-    if (!p) return false;
-    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    for (int i = 0; i < used_; i++)
-      if (lo_[i] <= a && a < hi_[i]) return true;
-    hipPointerAttribute_t attr;
-    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-      (void)hipGetLastError();
-      return false;
-    }
-    const bool dev = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
-    if (dev) {
-      hipDeviceptr_t base = nullptr;
-      size_t size = 0;
-      if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && size) {
-        lo_[next_] = reinterpret_cast<uintptr_t>(base);
-        hi_[next_] = lo_[next_] + size;
-        next_ = (next_ + 1) % kSlots;
-        used_ = std::min(used_ + 1, kSlots);
-      } else {
-        (void)hipGetLastError();
-      }
-    }
-    return dev;
-  }
-
- private:
-  static constexpr int kSlots = 128;  // a batch's inputs are often separate allocations
-  uintptr_t lo_[kSlots] = {}, hi_[kSlots] = {};
-  int used_ = 0, next_ = 0;
-};
 
 // Signals per GPU batch: bounded by workspace and, for host inputs, by the
 // staging buffer (whole signals are copied to the device).

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <set>
 
 #include "internal.hpp"
 
@@ -21,6 +22,22 @@ const char *last_error() { return g_last_error.c_str(); }
       return -1;                                                            \
     }                                                                       \
   } while (0)
+
+// ---------------------------------------------------------------- LDS limit
+void raise_lds_limit(const void *kernel)
+{
+  static std::mutex mu;
+  static std::set<std::pair<int, const void *>> done;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!done.insert({dev, kernel}).second) return;
+  int lim = 0;
+  if (hipDeviceGetAttribute(&lim, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess ||
+      lim <= 0)
+    lim = 160 * 1024;
+  (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+}
 
 // ---------------------------------------------------------------- RNG
 int gcd_int(int a, int b)
@@ -136,6 +153,24 @@ static std::vector<float2> to_f2(const std::vector<cd> &v, size_t cnt)
   return o;
 }
 
+// Taps by bucket: t[b * R + k] = tap[b + k*B] for b + k*B < w (zero past
+// the last tap), R = ceil(w / B).  The fold reads a bucket's taps from one
+// contiguous run.
+template <typename T2>
+static std::vector<T2> taps_by_bucket(const std::vector<cd> &v, int B, int &R)
+{
+  const int w = (int)v.size();
+  R = (w + B - 1) / B;
+  std::vector<T2> o((size_t)B * R);
+  for (auto &t : o) t.x = t.y = 0;
+  for (int i = 0; i < w; i++) {
+    T2 &t = o[(size_t)(i % B) * R + i / B];
+    t.x = (decltype(t.x))v[i].real();
+    t.y = (decltype(t.y))v[i].imag();
+  }
+  return o;
+}
+
 // Estimation window of one filter: win[d + m/2] = freq[(n - d) mod n] for
 // d = -m/2..m/2 (m = n/B); estimate_values reads only these
 // (src/computefourier-1.0-2.0.cc:371-385).
@@ -179,8 +214,8 @@ int64_t Plan::output_capacity() const
 
 int64_t Plan::plan_bytes() const
 {
-  return (int64_t)(taps_loc.bytes + taps_est.bytes + taps_loc32.bytes +
-                   taps_est32.bytes + win_loc.bytes + win_est.bytes +
+  return (int64_t)(taps_loc.bytes + taps_est.bytes + tapsT_loc32.bytes +
+                   tapsT_est32.bytes + win_loc.bytes + win_est.bytes +
                    tw64.bytes + tw32.bytes + v3_time1.bytes + v3_time2.bytes +
                    v3_freq1.bytes + v3_freq2.bytes);
 }
@@ -337,8 +372,8 @@ Plan *make_plan(int n, int k, sfft_version v)
   P->tw_nmax = nmax;
   bool ok = upload(P->taps_loc, to_d2(fl.time, fl.time.size())) &&
             upload(P->taps_est, to_d2(fe.time, fe.time.size())) &&
-            upload(P->taps_loc32, to_f2(fl.time, fl.time.size())) &&
-            upload(P->taps_est32, to_f2(fe.time, fe.time.size())) &&
+            upload(P->tapsT_loc32, taps_by_bucket<float2>(fl.time, p.B_loc, P->tR_loc)) &&
+            upload(P->tapsT_est32, taps_by_bucket<float2>(fe.time, p.B_est, P->tR_est)) &&
             upload(P->win_loc, est_window(fl.freq, p.n, p.B_loc)) &&
             upload(P->win_est, est_window(fe.freq, p.n, p.B_est)) &&
             upload(P->tw64, to_d2(tw, tw.size())) &&
@@ -390,8 +425,11 @@ Plan *clone_plan(const Plan *src, Workspace *ws, hipStream_t stream)
   P->owns_tables = false;
   P->taps_loc = src->taps_loc;
   P->taps_est = src->taps_est;
-  P->taps_loc32 = src->taps_loc32;
-  P->taps_est32 = src->taps_est32;
+  P->tapsT_loc32 = src->tapsT_loc32;
+  P->tapsT_est32 = src->tapsT_est32;
+  P->tR_loc = src->tR_loc;
+  P->tR_est = src->tR_est;
+  P->seg_rows = src->seg_rows;
   P->win_loc = src->win_loc;
   P->win_est = src->win_est;
   P->tw64 = src->tw64;
@@ -418,7 +456,7 @@ void free_plan(Plan *P)
   // a clone's stream belongs to its server worker, which waits for it
   if (P->stream && (P->own_stream || P->owns_tables)) (void)hipStreamSynchronize(P->stream);
   if (P->owns_tables) {
-    DevBuf *bufs[] = {&P->taps_loc, &P->taps_est, &P->taps_loc32, &P->taps_est32,
+    DevBuf *bufs[] = {&P->taps_loc, &P->taps_est, &P->tapsT_loc32, &P->tapsT_est32,
                       &P->win_loc, &P->win_est, &P->tw64, &P->tw32,
                       &P->v3_time1, &P->v3_time2, &P->v3_freq1, &P->v3_freq2};
     for (DevBuf *b : bufs) b->release();
@@ -782,8 +820,11 @@ int exec_batch(Plan *P, BatchIO &io)
   a.params = P->ws->params.as<const int>();
   a.taps_loc = P->taps_loc.as<const double2>();
   a.taps_est = P->taps_est.as<const double2>();
-  a.taps_loc32 = P->taps_loc32.as<const float2>();
-  a.taps_est32 = P->taps_est32.as<const float2>();
+  a.tapsT_loc32 = P->tapsT_loc32.as<const float2>();
+  a.tapsT_est32 = P->tapsT_est32.as<const float2>();
+  a.tR_loc = P->tR_loc;
+  a.tR_est = P->tR_est;
+  a.seg_rows = P->seg_rows;
   a.win_loc = P->win_loc.as<const double2>();
   a.win_est = P->win_est.as<const double2>();
   a.tw64 = P->tw64.as<const double2>();

@@ -25,6 +25,56 @@ constexpr int kMaxLoops = 32;
 
 void set_error(const std::string &msg);
 
+// Lets `kernel` launch with up to the device's opt-in LDS per workgroup
+// (160 KiB on gfx950).  The limit is raised once per kernel and device and
+// never lowered, so launches from concurrent threads cannot undercut each
+// other.  Kernels call it before every launch above the default 64 KiB.
+void raise_lds_limit(const void *kernel);
+
+// Device-pointer test with a cache of the allocation ranges met during ONE
+// API call (a batch of rows of one tensor costs one query).  The cache must
+// not outlive the call: once the caller frees an allocation, its address
+// range can be handed out again as host memory, and a stale entry would
+// make the kernels dereference a host address.
+class PtrKinds {
+ public:
+  bool is_device(const void *p) { return covers(p, 1); }
+
+  // p is device (or managed) memory and its allocation holds [p, p + bytes).
+  bool covers(const void *p, size_t bytes)
+  {
+    if (!p) return false;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    for (int i = 0; i < used_; i++)
+      if (lo_[i] <= a && a < hi_[i]) return bytes <= hi_[i] - a;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    const bool dev = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+    if (dev) {
+      hipDeviceptr_t base = nullptr;
+      size_t size = 0;
+      if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && size) {
+        lo_[next_] = reinterpret_cast<uintptr_t>(base);
+        hi_[next_] = lo_[next_] + size;
+        const bool fits = bytes <= hi_[next_] - a;
+        next_ = (next_ + 1) % kSlots;
+        used_ = std::min(used_ + 1, kSlots);
+        return fits;
+      }
+      (void)hipGetLastError();
+    }
+    return dev;
+  }
+
+ private:
+  static constexpr int kSlots = 128;  // a batch's inputs are often separate allocations
+  uintptr_t lo_[kSlots] = {}, hi_[kSlots] = {};
+  int used_ = 0, next_ = 0;
+};
+
 // ---------------------------------------------------------------- params
 // sFFT v1/v2 sizes (src/sfft.cc:298-372 with src/parameters.cc).
 struct V12Params {
@@ -131,7 +181,9 @@ struct Plan {
 
   // v1/v2 device tables
   DevBuf taps_loc, taps_est;      // double2[w]
-  DevBuf taps_loc32, taps_est32;  // float2[w]
+  DevBuf tapsT_loc32, tapsT_est32;  // float2[B * ceil(w/B)]: FP32 taps by bucket
+  int tR_loc = 0, tR_est = 0;       // ceil(w/B) of each filter
+  int seg_rows = 256;               // rows per chunk of the FP32 fold (sum order)
   DevBuf win_loc, win_est;        // double2[m+1]: freq[(n-d) mod n], d = -m/2..m/2
   DevBuf tw64, tw32;              // e^{-2 pi i j / Nmax}, j < Nmax/2
   int tw_nmax = 0;
@@ -226,7 +278,10 @@ struct V12Launch {
   const double2 *const *x;        // device array of S signal pointers
   const int *params;              // S x (2L + comb_loops): a[L], ai[L], off[cl]
   const double2 *taps_loc, *taps_est;
-  const float2 *taps_loc32, *taps_est32;
+  // FP32 taps by bucket: tapsT[b * tR + k] = tap[b + k*B] (k < tR = ceil(w/B))
+  const float2 *tapsT_loc32, *tapsT_est32;
+  int tR_loc, tR_est;
+  int seg_rows;    // rows per chunk of the FP32 fold (fixes its summation order)
   const double2 *win_loc, *win_est;
   const double2 *tw64;
   const float2 *tw32;

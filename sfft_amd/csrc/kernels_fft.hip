@@ -175,10 +175,8 @@ int radix2(T2 *data, T2 *tmp, int N, int C, int64_t nb_outer, int nb_inner, int6
   const int L = std::min(logN, logC);
   const size_t lds = ((size_t)1 << L) * sizeof(T2);
   if (lds > 65536) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_lds_fft_batch<T2>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_r2_runs<T2>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    raise_lds_limit(reinterpret_cast<const void *>(k_lds_fft_batch<T2>));
+    raise_lds_limit(reinterpret_cast<const void *>(k_r2_runs<T2>));
   }
   if (logN <= logC) {
     if (nb >= (1LL << 31)) return -1;
@@ -279,9 +277,7 @@ int batch_fft64(double2 *data, double2 *tmp, int N, int64_t nb_outer, int nb_inn
     int logN = 0;
     while ((1 << logN) < N) logN++;
     const size_t lds = (size_t)N * sizeof(double2);
-    if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_lds_fft_batch<double2>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (lds > 65536) raise_lds_limit(reinterpret_cast<const void *>(k_lds_fft_batch<double2>));
     hipLaunchKernelGGL(k_lds_fft_batch<double2>, dim3((unsigned)(nb_outer * nb_inner)),
                        dim3(kBlock), lds, st, data, logN, nb_inner, so, si, tw, tw_nmax / N);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -299,9 +295,7 @@ int batch_fft32(float2 *data, float2 *tmp, int N, int64_t nb_outer, int nb_inner
     int logN = 0;
     while ((1 << logN) < N) logN++;
     const size_t lds = (size_t)N * sizeof(float2);
-    if (lds > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_lds_fft_batch<float2>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (lds > 65536) raise_lds_limit(reinterpret_cast<const void *>(k_lds_fft_batch<float2>));
     hipLaunchKernelGGL(k_lds_fft_batch<float2>, dim3((unsigned)(nb_outer * nb_inner)),
                        dim3(kBlock), lds, st, data, logN, nb_inner, so, si, tw, tw_nmax / N);
     return hipGetLastError() == hipSuccess ? 0 : -1;

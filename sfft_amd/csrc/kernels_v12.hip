@@ -158,7 +158,8 @@ __global__ void __launch_bounds__(kBlock) k_comb_merge(V12Launch a)
 // ---------------------------------------------------------------- fold + FFT
 // One block per (signal, loop).  Each thread owns buckets m = tid + q*T and
 // sums, in ascending tap order like the reference, x[(i*ai) mod n]*tap[i]
-// over the taps i = m + r*B < w.  LDS: B double2 (or float2).
+// over the taps i = m + r*B < w.  LDS: B double2.  FP64 only (the FP32
+// form folds in k_fold_chunks).
 constexpr int kQ = 8;  // buckets per thread in flight (independent gathers)
 
 __global__ void __launch_bounds__(kBlock) k_fold_fft64(V12Launch a, int l0, int nl)
@@ -216,83 +217,15 @@ __global__ void __launch_bounds__(kBlock) k_fold_fft64(V12Launch a, int l0, int 
   for (int j = threadIdx.x; j < B; j += T) out[j] = smem64[j];
 }
 
-// FP32 bucketization: products in FP32, compensated (2Sum) FP32 sums.
-__global__ void __launch_bounds__(kBlock) k_fold_fft32(V12Launch a, int l0, int nl)
-{
-  extern __shared__ float2 smem32[];
-  const int s = blockIdx.x / nl, l = l0 + blockIdx.x % nl;
-  const bool isloc = l < a.Lloc;
-  const int B = isloc ? a.B_loc : a.B_est;
-  const int w = isloc ? a.w_loc : a.w_est;
-  const float2 *__restrict__ taps = isloc ? a.taps_loc32 : a.taps_est32;
-  const int stride = 2 * a.L + max(a.comb_loops, 1);
-  const unsigned ai = (unsigned)a.params[(int64_t)s * stride + a.L + l];
-  const double2 *__restrict__ x = a.x[s];
-  const unsigned mask = (unsigned)a.n - 1;
-  const int logB = ilog2(B);
-  const int T = blockDim.x;
-  const int R = (w + B - 1) / B;
-  for (int mb = threadIdx.x; mb < B; mb += T * kQ) {
-    float2 sum[kQ], cmp[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; q++) {
-      sum[q] = make_float2(0.f, 0.f);
-      cmp[q] = make_float2(0.f, 0.f);
-    }
-    for (int r = 0; r < R; r++) {
-      double2 xv[kQ];
-      float2 tv[kQ];
-#pragma unroll
-      for (int q = 0; q < kQ; q++) {
-        const int m = mb + q * T;
-        const int i = m + r * B;
-        if (m < B && i < w) {
-          xv[q] = x[((unsigned)i * ai) & mask];
-          tv[q] = taps[i];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kQ; q++) {
-        const int m = mb + q * T;
-        const int i = m + r * B;
-        if (m < B && i < w) {
-          const float2 xf = make_float2((float)xv[q].x, (float)xv[q].y);
-          const float2 p = cmul(xf, tv[q]);
-          // 2Sum: t = s + p exactly split into t and err
-          float t = sum[q].x + p.x, bp = t - sum[q].x;
-          cmp[q].x += (sum[q].x - (t - bp)) + (p.x - bp);
-          sum[q].x = t;
-          t = sum[q].y + p.y;
-          bp = t - sum[q].y;
-          cmp[q].y += (sum[q].y - (t - bp)) + (p.y - bp);
-          sum[q].y = t;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kQ; q++) {
-      const int m = mb + q * T;
-      if (m < B)
-        smem32[bitrev(m, logB)] =
-            make_float2(sum[q].x + cmp[q].x, sum[q].y + cmp[q].y);
-    }
-  }
-  __syncthreads();
-  lds_fft(smem32, logB, a.tw32, a.tw_nmax / B);
-  float2 *out = a.xs32 + (int64_t)s * a.xs +
-                loop_offset(l, a.Lloc, a.B_loc, a.B_est);
-  for (int j = threadIdx.x; j < B; j += T) out[j] = smem32[j];
-}
-
 // ---------------------------------------------------------------- streamed fold
-// The same bucket sums, scheduled for the Infinity Cache (MALL): work items
+// The FP64 bucket sums, scheduled for the Infinity Cache (MALL): work items
 // are (signal, loop, 64 buckets), one per wavefront, numbered signal-major,
 // and a fixed set of resident wavefronts walks them in order.  Only a few
 // signals are then in flight at once, so the second and later touches of a
 // signal's 128-byte lines (20 loops gather ~20% of its samples) find them in
 // the MALL instead of HBM.  Each lane sums its bucket in ascending tap order
 // (identical arithmetic to the fused kernel); the B-point FFT follows in
-// the bucket FFT batches.  Buckets are written pre-FFT into xs64/xs32.
+// the bucket FFT batches.  Buckets are written pre-FFT into xs64.
 constexpr int kFoldWaves = 4;  // wavefronts per block
 
 struct FoldItems {
@@ -317,7 +250,6 @@ __host__ __device__ inline FoldItems fold_items(const V12Launch &a, int l0, int 
   return f;
 }
 
-template <bool FP32>
 __global__ void __launch_bounds__(64 * kFoldWaves) k_fold_stream(V12Launch a, FoldItems f)
 {
   const int lane = threadIdx.x & 63;
@@ -346,64 +278,134 @@ __global__ void __launch_bounds__(64 * kFoldWaves) k_fold_stream(V12Launch a, Fo
     const double2 *__restrict__ x = a.x[s];
     const int64_t out = (int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est) + m;
     const int R = (w - m + B - 1) / B;  // taps m, m+B, ... below w
-    if (!FP32) {
-      const double2 *__restrict__ taps = isloc ? a.taps_loc : a.taps_est;
-      double2 acc = make_double2(0.0, 0.0);
-      int r = 0;
-      for (; r + 8 <= R; r += 8) {
-        double2 xv[8], tv[8];
+    const double2 *__restrict__ taps = isloc ? a.taps_loc : a.taps_est;
+    double2 acc = make_double2(0.0, 0.0);
+    int r = 0;
+    for (; r + 8 <= R; r += 8) {
+      double2 xv[8], tv[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const unsigned i = (unsigned)(m + (r + q) * B);
-          xv[q] = x[(i * ai) & mask];
-          tv[q] = taps[i];
-        }
+      for (int q = 0; q < 8; q++) {
+        const unsigned i = (unsigned)(m + (r + q) * B);
+        xv[q] = x[(i * ai) & mask];
+        tv[q] = taps[i];
+      }
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          acc.x = acc.x + (xv[q].x * tv[q].x - xv[q].y * tv[q].y);
-          acc.y = acc.y + (xv[q].x * tv[q].y + xv[q].y * tv[q].x);
-        }
+      for (int q = 0; q < 8; q++) {
+        acc.x = acc.x + (xv[q].x * tv[q].x - xv[q].y * tv[q].y);
+        acc.y = acc.y + (xv[q].x * tv[q].y + xv[q].y * tv[q].x);
       }
-      for (; r < R; r++) {
-        const unsigned i = (unsigned)(m + r * B);
-        const double2 xv = x[(i * ai) & mask], tv = taps[i];
-        acc.x = acc.x + (xv.x * tv.x - xv.y * tv.y);
-        acc.y = acc.y + (xv.x * tv.y + xv.y * tv.x);
-      }
-      a.xs64[out] = acc;
-    } else {
-      const float2 *__restrict__ taps = isloc ? a.taps_loc32 : a.taps_est32;
-      float2 sum = make_float2(0.f, 0.f), cmp = make_float2(0.f, 0.f);
-      auto add = [&](double2 xv, float2 tv) {
-        const float2 p = cmul(make_float2((float)xv.x, (float)xv.y), tv);
-        float t = sum.x + p.x, bp = t - sum.x;
-        cmp.x += (sum.x - (t - bp)) + (p.x - bp);
-        sum.x = t;
-        t = sum.y + p.y;
-        bp = t - sum.y;
-        cmp.y += (sum.y - (t - bp)) + (p.y - bp);
-        sum.y = t;
-      };
-      int r = 0;
-      for (; r + 8 <= R; r += 8) {
-        double2 xv[8];
-        float2 tv[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const unsigned i = (unsigned)(m + (r + q) * B);
-          xv[q] = x[(i * ai) & mask];
-          tv[q] = taps[i];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) add(xv[q], tv[q]);
-      }
-      for (; r < R; r++) {
-        const unsigned i = (unsigned)(m + r * B);
-        add(x[(i * ai) & mask], taps[i]);
-      }
-      a.xs32[out] = make_float2(sum.x + cmp.x, sum.y + cmp.y);
     }
+    for (; r < R; r++) {
+      const unsigned i = (unsigned)(m + r * B);
+      const double2 xv = x[(i * ai) & mask], tv = taps[i];
+      acc.x = acc.x + (xv.x * tv.x - xv.y * tv.y);
+      acc.y = acc.y + (xv.x * tv.y + xv.y * tv.x);
+    }
+    a.xs64[out] = acc;
   }
+}
+
+// ---------------------------------------------------------------- chunked fold
+// The FP32 form's bucket sums, grouped so that an XCD's L2 serves repeated
+// touches of a 128-byte line.  Loop l adds x[p] * tap[i] into bucket
+// b = i mod B for the taps i = b + kB < w at p = i*ai mod n; p mod B =
+// b*ai mod B, so bucket b only ever reads residue class r = b*ai mod B, and
+// b = r*a mod B (a*ai = 1 mod n).  A workgroup takes the buckets of kResPer
+// consecutive residues in every loop of the launch, one (residue, loop) pair
+// per thread: it is the only reader of those residues' lines (viewing the
+// signal as n/B rows of B samples, p = r + t*B).  Rows are walked in chunks
+// of 2^seg_log rows; each thread adds, chunk after chunk, its taps whose
+// sample lies in the chunk, in ascending tap order, with compensated (2Sum)
+// FP32 sums, and adds the chunks' sums in chunk order.  The workgroup's
+// threads work on the same chunk at about the same time, so the touches of
+// one line (taps of different loops in one row) fall within one memory round
+// trip and the L2 serves all but the first.  The pairs' taps are staged in
+// LDS by coalesced loads of their by-bucket runs.  This fixes the FP32 form's
+// summation order; with one chunk it is plain ascending tap order.
+constexpr int kResPer = 16;                        // residues per workgroup
+constexpr int kFoldResBlock = kResPer * kMaxLoops;  // 512: one pair per thread
+constexpr int kChunkBatch = 8;                     // gathers in flight per thread and round
+
+struct FoldChunks {
+  int l0, nl;        // loops [l0, l0 + nl), all with B buckets
+  int B, logB;
+  int res, groups;   // residues per workgroup, B / res
+  int seg_log, nseg; // rows per chunk 2^seg_log, chunks
+  int tR;            // LDS stride of a pair's taps (max ceil(w/B) of the loops)
+};
+
+__global__ void __launch_bounds__(kFoldResBlock) k_fold_chunks(V12Launch a, FoldChunks f)
+{
+  extern __shared__ float2 s_tap[];  // pair-major, f.tR taps each
+  __shared__ int s_b[kFoldResBlock];
+  const int g = (int)(blockIdx.x % (unsigned)f.groups);
+  const int s = (int)(blockIdx.x / (unsigned)f.groups);
+  const int tid = threadIdx.x;
+  const int npairs = f.res * f.nl;
+  const bool has = tid < npairs;
+  const int e = tid % f.res, jl = has ? tid / f.res : 0;
+  const int l = f.l0 + jl;
+  const bool isloc = l < a.Lloc;
+  const int *prm = a.params + (int64_t)s * (2 * a.L + max(a.comb_loops, 1));
+  const unsigned av = (unsigned)prm[l], ai = (unsigned)prm[a.L + l];
+  const int B = f.B, logB = f.logB;
+  const unsigned mask = (unsigned)a.n - 1;
+  const int b = (int)(((unsigned)(g * f.res + e) * av) & (unsigned)(B - 1));
+  const int w = isloc ? a.w_loc : a.w_est;
+  const int R = (has && b < w) ? (w - b + B - 1) / B : 0;
+  const unsigned p0 = ((unsigned)b * ai) & mask;
+  const unsigned step = ((unsigned)B * ai) & mask;
+  s_b[tid] = b;
+  __syncthreads();
+  for (int idx = tid; idx < npairs * f.tR; idx += blockDim.x) {
+    const int pr = idx / f.tR, k = idx - pr * f.tR;
+    const bool lp = f.l0 + pr / f.res < a.Lloc;
+    const int tRp = lp ? a.tR_loc : a.tR_est;
+    const float2 *T = lp ? a.tapsT_loc32 : a.tapsT_est32;
+    s_tap[idx] = k < tRp ? T[(int64_t)s_b[pr] * tRp + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  if (!has) return;  // no barrier below
+  const float2 *mytap = s_tap + tid * f.tR;
+  const double2 *__restrict__ x = a.x[s];
+  const int cshift = logB + f.seg_log;
+  float2 tot = make_float2(0.f, 0.f);
+  for (int c = 0; c < f.nseg; c++) {
+    unsigned m = 0, pp = p0;
+    for (int k = 0; k < R; k++) {
+      if ((int)(pp >> cshift) == c) m |= 1u << k;
+      pp = (pp + step) & mask;
+    }
+    float2 sum = make_float2(0.f, 0.f), cmp = make_float2(0.f, 0.f);
+    while (m) {
+      double2 xv[kChunkBatch];
+      int kk[kChunkBatch];
+#pragma unroll
+      for (int q = 0; q < kChunkBatch; q++) {
+        kk[q] = m ? __ffs(m) - 1 : -1;
+        if (kk[q] >= 0) {
+          xv[q] = x[(p0 + (unsigned)kk[q] * step) & mask];
+          m &= m - 1;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kChunkBatch; q++) {
+        if (kk[q] >= 0) {
+          const float2 pr = cmul(make_float2((float)xv[q].x, (float)xv[q].y), mytap[kk[q]]);
+          float t = sum.x + pr.x, bp = t - sum.x;
+          cmp.x += (sum.x - (t - bp)) + (pr.x - bp);
+          sum.x = t;
+          t = sum.y + pr.y;
+          bp = t - sum.y;
+          cmp.y += (sum.y - (t - bp)) + (pr.y - bp);
+          sum.y = t;
+        }
+      }
+    }
+    const float2 part = make_float2(sum.x + cmp.x, sum.y + cmp.y);
+    tot = c == 0 ? part : make_float2(tot.x + part.x, tot.y + part.y);
+  }
+  a.xs32[(int64_t)s * a.xs + loop_offset(l, a.Lloc, a.B_loc, a.B_est) + b] = tot;
 }
 
 // ---------------------------------------------------------------- v1 votes
@@ -1081,14 +1083,11 @@ __global__ void k_zero(double2 *out, int64_t n)
 }
 
 // Kernels whose dynamic LDS can exceed the default 64 KiB opt in to the
-// gfx950 maximum once.
+// device maximum (once per kernel; see raise_lds_limit).
 template <typename K>
 void allow_lds(K kernel, size_t bytes)
 {
-  if (bytes > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
+  if (bytes > 65536) raise_lds_limit(reinterpret_cast<const void *>(kernel));
 }
 
 int launched(const char *what)
@@ -1148,84 +1147,120 @@ int comb(const V12Launch &a, hipStream_t st)
   return 0;
 }
 
+// Loops of one bucket count folded by k_fold_chunks.
+static int fold_chunks(const V12Launch &a, hipStream_t st, int l0, int nl, int B)
+{
+  FoldChunks f;
+  f.l0 = l0;
+  f.nl = nl;
+  f.B = B;
+  f.logB = 31 - __builtin_clz((unsigned)B);
+  f.res = std::min(kResPer, B);
+  f.groups = B / f.res;
+  const int rows = a.n / B;
+  const int seg = std::min(rows, a.seg_rows);
+  f.seg_log = 31 - __builtin_clz((unsigned)seg);
+  f.nseg = rows / seg;
+  f.tR = 0;
+  if (l0 < a.Lloc) f.tR = std::max(f.tR, a.tR_loc);
+  if (l0 + nl > a.Lloc) f.tR = std::max(f.tR, a.tR_est);
+  if (f.tR > 32) {  // tap masks are 32 bits; tolerances >= 5e-9 give at most 27 taps
+    set_error("fold: more than 32 taps per bucket");
+    return -1;
+  }
+  const int threads = (f.res * nl + 63) / 64 * 64;
+  const size_t lds = (size_t)f.res * nl * f.tR * sizeof(float2);
+  allow_lds(k_fold_chunks, lds);
+  const dim3 grid((unsigned)((int64_t)a.S * f.groups));
+  hipLaunchKernelGGL(k_fold_chunks, grid, dim3(threads), lds, st, a, f);
+  return launched("fold_chunks");
+}
+
 int fold_fft(const V12Launch &a, hipStream_t st, int part)
 {
-  // loops [0, n64) in FP64, [n64, L) in FP32
+  // loops [0, n64) in FP64, [n64, L) in FP32 (folded by k_fold_chunks, with
+  // the FFTs in part 1 whatever the schedule)
   const int n64 = !a.fp32 ? a.L : (a.loc64 ? a.Lloc : 0);
   const int B64 = n64 == 0 ? 0 : (n64 > a.Lloc ? max(a.B_loc, a.B_est) : a.B_loc);
-  const int B32 = n64 >= a.L ? 0 : (n64 == 0 ? max(a.B_loc, a.B_est) : a.B_est);
   int launches = 0;
-  if (a.fold_mode >= 1) {
-    const dim3 blk(64 * kFoldWaves);
-    if (part == 0) {
-      if (n64 > 0) {
-        const FoldItems f = fold_items(a, 0, n64);
-        const int64_t g =
-            std::min<int64_t>(a.stream_blocks, (f.total + kFoldWaves - 1) / kFoldWaves);
-        hipLaunchKernelGGL(k_fold_stream<false>, dim3((unsigned)g), blk, 0, st, a, f);
-        if (launched("fold_stream64") != 0) return -1;
-        launches++;
-      }
-      if (n64 < a.L) {
-        const FoldItems f = fold_items(a, n64, a.L - n64);
-        const int64_t g =
-            std::min<int64_t>(a.stream_blocks, (f.total + kFoldWaves - 1) / kFoldWaves);
-        hipLaunchKernelGGL(k_fold_stream<true>, dim3((unsigned)g), blk, 0, st, a, f);
-        if (launched("fold_stream32") != 0) return -1;
-        launches++;
-      }
-    } else if (a.rows_layout) {
-      // v2: every loop in one precision, written in the alias-row layout
-      const int Bm = max(a.B_loc, a.B_est);
-      if (a.fp32) {
-        allow_lds(k_bucket_fft_rows<true>, (size_t)Bm * sizeof(float2));
-        hipLaunchKernelGGL(k_bucket_fft_rows<true>, dim3(a.S * a.L), dim3(kBlock),
-                           (size_t)Bm * sizeof(float2), st, a, 0, a.L);
-      } else {
-        allow_lds(k_bucket_fft_rows<false>, (size_t)Bm * sizeof(double2));
-        hipLaunchKernelGGL(k_bucket_fft_rows<false>, dim3(a.S * a.L), dim3(kBlock),
-                           (size_t)Bm * sizeof(double2), st, a, 0, a.L);
-      }
-      if (launched("bucket_fft_rows") != 0) return -1;
+  if (part == 0) {
+    if (n64 > 0 && a.fold_mode >= 1) {
+      const FoldItems f = fold_items(a, 0, n64);
+      const int64_t g =
+          std::min<int64_t>(a.stream_blocks, (f.total + kFoldWaves - 1) / kFoldWaves);
+      hipLaunchKernelGGL(k_fold_stream, dim3((unsigned)g), dim3(64 * kFoldWaves), 0, st, a, f);
+      if (launched("fold_stream64") != 0) return -1;
       launches++;
-    } else {
-      // the location and estimation loops as two batches (their bucket
-      // counts may differ): in LDS up to the LDS size, else radix-2 runs
-      // plus global passes
-      int groups[2][3] = {{0, a.Lloc, a.B_loc}, {a.Lloc, a.L, a.B_est}};
-      if (a.B_loc == a.B_est && (n64 == 0 || n64 == a.L)) {  // one batch
-        groups[0][1] = a.L;
-        groups[1][0] = a.L;  // empty
-      }
-      for (const auto &g : groups) {
-        const int l0 = g[0], l1 = g[1], B = g[2];
-        if (l1 <= l0) continue;
-        const int64_t off = loop_offset_host(l0, a.Lloc, a.B_loc, a.B_est);
-        // a group never straddles n64: it is either Lloc, L or 0
-        const int rc = l0 < n64
-                           ? radix2_fft64(a.xs64 + off, a.tmp64 ? a.tmp64 + off : nullptr, B,
-                                          a.S, l1 - l0, a.xs, B, a.tw64, a.tw_nmax, st)
-                           : radix2_fft32(a.xs32 + off, a.tmp32 ? a.tmp32 + off : nullptr, B,
-                                          a.S, l1 - l0, a.xs, B, a.tw32, a.tw_nmax, st);
-        if (rc != 0) return launch_failed("bucket FFT");
+    } else if (n64 > 0) {
+      allow_lds(k_fold_fft64, (size_t)B64 * sizeof(double2));
+      hipLaunchKernelGGL(k_fold_fft64, dim3(a.S * n64), dim3(kBlock),
+                         (size_t)B64 * sizeof(double2), st, a, 0, n64);
+      if (launched("fold_fft64") != 0) return -1;
+      launches++;
+    }
+    if (n64 < a.L) {
+      // the FP32 loops, in runs of one bucket count
+      auto run = [&](int l0, int nl, int B) { return fold_chunks(a, st, l0, nl, B); };
+      if (n64 < a.Lloc && a.B_loc != a.B_est) {
+        if (run(n64, a.Lloc - n64, a.B_loc) != 0 || run(a.Lloc, a.L - a.Lloc, a.B_est) != 0)
+          return -1;
+        launches += 2;
+      } else {
+        if (run(n64, a.L - n64, n64 < a.Lloc ? a.B_loc : a.B_est) != 0) return -1;
         launches++;
       }
     }
     return launches;
   }
-  if (part == 1) return 0;  // fused into the fold
-  if (n64 > 0) {
-    allow_lds(k_fold_fft64, (size_t)B64 * sizeof(double2));
-    hipLaunchKernelGGL(k_fold_fft64, dim3(a.S * n64), dim3(kBlock),
-                       (size_t)B64 * sizeof(double2), st, a, 0, n64);
-    if (launched("fold_fft64") != 0) return -1;
-    launches++;
+  // part 1: the bucket FFTs of every loop not fused into its fold
+  if (a.rows_layout) {
+    // v2: every loop in one precision, written in the alias-row layout
+    const int Bm = max(a.B_loc, a.B_est);
+    if (a.fp32) {
+      allow_lds(k_bucket_fft_rows<true>, (size_t)Bm * sizeof(float2));
+      hipLaunchKernelGGL(k_bucket_fft_rows<true>, dim3(a.S * a.L), dim3(kBlock),
+                         (size_t)Bm * sizeof(float2), st, a, 0, a.L);
+    } else {
+      allow_lds(k_bucket_fft_rows<false>, (size_t)Bm * sizeof(double2));
+      hipLaunchKernelGGL(k_bucket_fft_rows<false>, dim3(a.S * a.L), dim3(kBlock),
+                         (size_t)Bm * sizeof(double2), st, a, 0, a.L);
+    }
+    if (launched("bucket_fft_rows") != 0) return -1;
+    return 1;
   }
-  if (n64 < a.L) {
-    allow_lds(k_fold_fft32, (size_t)B32 * sizeof(float2));
-    hipLaunchKernelGGL(k_fold_fft32, dim3(a.S * (a.L - n64)), dim3(kBlock),
-                       (size_t)B32 * sizeof(float2), st, a, n64, a.L - n64);
-    if (launched("fold_fft32") != 0) return -1;
+  // the location and estimation loops as batches of one bucket count and
+  // precision: in LDS up to the LDS size, else radix-2 runs plus global
+  // passes; FP64 loops fused into their fold (schedule 0) are done
+  const int f0 = a.fold_mode == 0 ? n64 : 0;  // first loop still to transform
+  int groups[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  int ng = 0;
+  auto add_run = [&](int l0, int l1) {
+    // split [l0, l1) at Lloc (bucket counts) and n64 (precision)
+    int cuts[4] = {l0, std::min(std::max(a.Lloc, l0), l1), std::min(std::max(n64, l0), l1), l1};
+    std::sort(cuts, cuts + 4);
+    for (int c = 0; c < 3; c++)
+      if (cuts[c + 1] > cuts[c]) {
+        groups[ng][0] = cuts[c];
+        groups[ng][1] = cuts[c + 1];
+        groups[ng][2] = cuts[c] < a.Lloc ? a.B_loc : a.B_est;
+        ng++;
+      }
+  };
+  add_run(f0, a.L);
+  for (int gi = 0; gi < ng; gi++) {
+    int l0 = groups[gi][0], l1 = groups[gi][1];
+    const int B = groups[gi][2];
+    // merge with the next run when bucket count and precision agree
+    while (gi + 1 < ng && groups[gi + 1][2] == B && (groups[gi + 1][0] < n64) == (l0 < n64)) {
+      l1 = groups[++gi][1];
+    }
+    const int64_t off = loop_offset_host(l0, a.Lloc, a.B_loc, a.B_est);
+    const int rc = l0 < n64
+                       ? radix2_fft64(a.xs64 + off, a.tmp64 ? a.tmp64 + off : nullptr, B,
+                                      a.S, l1 - l0, a.xs, B, a.tw64, a.tw_nmax, st)
+                       : radix2_fft32(a.xs32 + off, a.tmp32 ? a.tmp32 + off : nullptr, B,
+                                      a.S, l1 - l0, a.xs, B, a.tw32, a.tw_nmax, st);
+    if (rc != 0) return launch_failed("bucket FFT");
     launches++;
   }
   return launches;

@@ -853,8 +853,7 @@ static int mansour_prepass(const V3Launch &a, hipStream_t st)
 
 static void allow_v3_lds(const void *kernel, size_t bytes)
 {
-  if (bytes > 65536)
-    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (bytes > 65536) raise_lds_limit(kernel);
 }
 
 int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,

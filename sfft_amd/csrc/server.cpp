@@ -492,6 +492,22 @@ int64_t sfft_server_submit(sfft_server *S, const sfft_request *r)
     set_error(why);
     return -1;
   }
+  // the kernels dereference these: host memory would fault the GPU
+  {
+    PtrKinds kinds;
+    const char *bad = nullptr;
+    if (!kinds.covers(r->in, (size_t)r->n * 16)) bad = "input is not device memory holding n values";
+    else if (r->freq && !kinds.covers(r->freq, (size_t)r->capacity * 4))
+      bad = "freq is not device memory holding capacity entries";
+    else if (r->val && !kinds.covers(r->val, (size_t)r->capacity * 16))
+      bad = "val is not device memory holding capacity entries";
+    if (bad) {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->st.rejected++;
+      set_error(bad);
+      return -1;
+    }
+  }
   Req *q = new Req;
   q->key = key_of(*r);
   q->r = *r;

@@ -147,6 +147,40 @@ def test_fp32_refine_matches_fp64(torch_cuda, version, n, k, snr):
     assert err.max() <= 1e-6, err.max()
 
 
+@pytest.mark.parametrize("version,n,k,snr", [
+    (2, 1 << 16, 50, None),      # 128 rows: one chunk (plain ascending tap order)
+    (2, 1 << 22, 64, 40.0),      # 2048 rows: eight chunks of 256
+    (1, 1 << 20, 50, None),      # v1: FP32 estimation loops only (B_est = 512)
+    (2, 1 << 20, 64, 30.0),
+    (1, 1 << 24, 50, None),      # B_est = 1024: 64 chunks
+])
+def test_fp32_buckets_bit_exact_model(torch_cuda, version, n, k, snr):
+    """The FP32 form's buckets after the bucket FFTs equal, bit for bit, the
+    model of its summation order (oracle/fp32_model.c) on the plan's taps
+    and draws."""
+    print(f"case fp32 buckets v{version} n={n} k={k} snr={snr}", flush=True)
+    x, f, v = signal_np(n, k, seed=4300 + k, snr_db=snr)
+    plan = _gpu_plan(version, n, k, precision="fp32_refine")
+    plan.execute_sparse_host(x)
+    got = plan.debug_buckets(0)
+    p = dict(zip(["B_loc", "B_est", "B_thresh", "W_comb", "comb_loops", "loc_loops",
+                  "est_loops", "thr_loops", "w_loc", "w_est"], plan.params()))
+    L, Lloc = p["loc_loops"] + p["est_loops"], p["loc_loops"]
+    oracle.seed_glibc(17, 4242)
+    a, ai, off = oracle.v12_draws(n, L, p["comb_loops"] if version == 2 else 0, p["W_comb"])
+    nmax = max(p["B_loc"], p["B_est"], p["W_comb"] if version == 2 else 0)
+    lt, _ = plan.filter(0)
+    et, _ = plan.filter(1)
+    first32 = 0 if version == 2 else Lloc  # v1 keeps its location loops in FP64
+    off_l = 0
+    for l in range(L):
+        B = p["B_loc"] if l < Lloc else p["B_est"]
+        if l >= first32:
+            want = oracle.fp32_buckets(x, n, B, lt if l < Lloc else et, ai[l], 256, nmax)
+            assert np.array_equal(got[off_l:off_l + B], want), (l, np.abs(got[off_l:off_l + B] - want).max())
+        off_l += B
+
+
 @pytest.mark.parametrize("version", [1, 2])
 def test_batch_equals_single(torch_cuda, version):
     """exec_many of a batch gives each signal the same result as alone,
@@ -224,7 +258,7 @@ def test_autotune_keeps_draws_and_results(torch_cuda):
     x, f, v = signal_np(n, k, seed=77, snr_db=30.0)
     tuned = _gpu_plan(2, n, k, "fp32_refine")
     mode, blocks = tuned.autotune(batch=8)
-    assert mode in (0, 1) and blocks >= 0
+    assert mode in (0, 1, 2) and blocks >= 0
     fresh = _gpu_plan(2, n, k, "fp32_refine")
     a = tuned.execute_sparse_host(x)
     b = fresh.execute_sparse_host(x)

@@ -13,8 +13,16 @@ import scipy.signal
 import oracle
 from sfft_amd.signals import signal_np
 
-REF = oracle.ref_helpers()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference tree)")
+
+@pytest.fixture(scope="module")
+def REF():
+    """The reference's own helpers (oracle/_ref), loaded only by the tests
+    that use them; oracle/_ref is built where the reference tree exists and
+    never travels with the GPU runs."""
+    ref = oracle.ref_helpers()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    return ref
 
 
 def _p(a):
@@ -22,8 +30,7 @@ def _p(a):
 
 
 # ---------------------------------------------------------------- _ref pins
-@needs_ref
-def test_gcd_mod_inverse_floor_pow2_vs_reference():
+def test_gcd_mod_inverse_floor_pow2_vs_reference(REF):
     rng = np.random.default_rng(0)
     L = oracle.lib()
     for n in [8192, 1 << 16, 1 << 20, 1 << 22, 1 << 24]:
@@ -39,9 +46,8 @@ def test_gcd_mod_inverse_floor_pow2_vs_reference():
         assert L.or_floor_to_pow2(x) == REF.ref_floor_to_pow2(x)
 
 
-@needs_ref
 @pytest.mark.parametrize("n,num", [(16, 3), (2048, 128), (4096, 128), (512, 100), (101, 100)])
-def test_find_largest_indices_vs_reference_with_ties(n, num):
+def test_find_largest_indices_vs_reference_with_ties(REF, n, num):
     rng = np.random.default_rng(n + num)
     for trial in range(20):
         if trial % 3 == 0:   # heavy ties around the cutoff
@@ -58,8 +64,7 @@ def test_find_largest_indices_vs_reference_with_ties(n, num):
         assert np.array_equal(got, ref)
 
 
-@needs_ref
-def test_radix_sort_vs_reference():
+def test_radix_sort_vs_reference(REF):
     rng = np.random.default_rng(3)
     a = rng.integers(0, 1 << 24, 5000).astype(np.int32)
     r = a.copy()
@@ -69,9 +74,8 @@ def test_radix_sort_vs_reference():
     assert np.array_equal(o, r)
 
 
-@needs_ref
 @pytest.mark.parametrize("with_comb", [0, 1])
-def test_parameter_tables_vs_reference(with_comb):
+def test_parameter_tables_vs_reference(REF, with_comb):
     L = oracle.lib()
     keys_n = [1 << p for p in range(13, 25)] + [12345]
     keys_k = [50, 100, 200, 500, 1000, 2000, 2500, 4000, 64]
@@ -285,3 +289,70 @@ def test_v3_noiseless_exact_support(n, k):
     extra = np.setdiff1d(loc, f)
     assert np.abs(r["out"][extra]).max(initial=0.0) < 1e-4
     assert np.abs(r["out"][f] - v).max() < 1e-4
+
+
+# ---------------------------------------------------------------- v3 plan sizes
+@pytest.mark.parametrize("n,k,expect", [
+    # (n, k): W_Man, B_g1, w_g1, B_g2, w_g2 -- SURVEY Appendix A, computed
+    # from src/sfft.cc:513-560 (Mancst 10, Bcst 1, Bcst2 0.25, tolerance 1e-8)
+    (1 << 24, 200, (1024, 128, 2433, 32, 607)),
+    (1 << 24, 500, (4096, 256, 6083, 64, 1521)),
+    (1 << 24, 1000, (8192, 512, 12167, 128, 3041)),
+    (1 << 20, 10, (64, 8, 121, 2, 23)),
+    (1 << 16, 10, (64, 8, 121, 2, 23)),
+])
+def test_v3_plan_sizes(n, k, expect):
+    p = oracle.v3_params(n, k)
+    assert (p.W_Man, p.B_g1, p.w_g1, p.B_g2, p.w_g2) == expect
+    # Gauss filters: b = n / B (gwidth 1), lobefrac = 0.5 / BB
+    assert p.b_g1 == n // p.B_g1 and p.b_g2 == n // p.B_g2
+
+
+def test_v3_w_man_capped_at_half_n():
+    """W_Man = min(n/2, floor_pow2(10 k)) (src/sfft.cc:527-528)."""
+    p = oracle.v3_params(1 << 12, 1000)
+    assert p.W_Man == 1 << 11
+
+
+# ---------------------------------------------------------------- median rank
+def test_estimate_is_lower_median_of_loop_values():
+    """estimate_values takes, per located frequency and separately for the
+    real and imaginary parts, the element of rank (L-1)/2 of the L per-loop
+    values bucket * conj(G) / |G|^2 (src/computefourier-1.0-2.0.cc:366-413).
+    Restated here in numpy from the oracle's own buckets (x_samp) and filter
+    responses; on a noisy signal the lower and upper medians differ for
+    nearly every frequency, so a rank of L/2 fails."""
+    n, k = 1 << 14, 50
+    x, _, _ = signal_np(n, k, seed=9, snr_db=10.0)
+    oracle.seed_glibc(17, 4242)
+    P = oracle.V12(n, k, True)
+    a, ai, off = P.draw()
+    tr = P.exec(x, a, ai, off, trace=True)
+    f = P.filters()
+    p = P.p
+    L, Lloc = P.loops, p.loc_loops
+    hits = np.unique(tr["hits"])
+    vals = np.zeros((len(hits), L), np.complex128)
+    off_l = 0
+    for l in range(L):
+        B = p.B_loc if l < Lloc else p.B_est
+        F = f["loc_freq"] if l < Lloc else f["est_freq"]
+        m = n // B
+        pi = (hits.astype(np.int64) * int(ai[l])) % n
+        bkt, dist = pi // m, pi % m
+        up = dist > m // 2
+        bkt = np.where(up, (bkt + 1) % B, bkt)
+        dist = np.where(up, dist - m, dist)
+        G = F[(n - dist) % n]
+        U = tr["x_samp"][off_l + bkt]
+        den = G.real * G.real + G.imag * G.imag
+        vals[:, l] = ((U.real * G.real + U.imag * G.imag) / den
+                      + 1j * ((U.real * G.imag - U.imag * G.real) / den))
+        off_l += B
+    assert L % 2 == 0
+    re, im = np.sort(vals.real, axis=1), np.sort(vals.imag, axis=1)
+    lo = (L - 1) // 2
+    got = tr["out"][hits]
+    assert np.array_equal(got.real, re[:, lo]) and np.array_equal(got.imag, im[:, lo])
+    # the test can tell the ranks apart
+    assert np.mean(re[:, lo] != re[:, L // 2]) > 0.9
