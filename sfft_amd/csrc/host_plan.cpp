@@ -3,7 +3,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <set>
 
 #include "internal.hpp"
 
@@ -26,17 +25,25 @@ const char *last_error() { return g_last_error.c_str(); }
 // ---------------------------------------------------------------- LDS limit
 void raise_lds_limit(const void *kernel)
 {
+  // a fixed table: nothing is allocated here, whoever calls
   static std::mutex mu;
-  static std::set<std::pair<int, const void *>> done;
+  static std::pair<int, const void *> done[512];
+  static int ndone = 0;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
   std::lock_guard<std::mutex> lk(mu);
-  if (!done.insert({dev, kernel}).second) return;
-  int lim = 0;
-  if (hipDeviceGetAttribute(&lim, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess ||
-      lim <= 0)
-    lim = 160 * 1024;
-  (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+  for (int i = 0; i < ndone; i++)
+    if (done[i].first == dev && done[i].second == kernel) return;
+  // gfx950: 160 KiB of LDS per workgroup, static and dynamic together
+  size_t stat = 0;
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, kernel) == hipSuccess) stat = fa.sharedSizeBytes;
+  else (void)hipGetLastError();
+  const int lim = (int)(160 * 1024 - std::min<size_t>(stat, 160 * 1024));
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess)
+    (void)hipGetLastError();  // the launch reports it
+  else if (ndone < 512)
+    done[ndone++] = {dev, kernel};
 }
 
 // ---------------------------------------------------------------- RNG
