@@ -109,6 +109,20 @@ int sfft_exec_sparse(sfft_plan *plan, const complex_t *in, sfft_output_c *out);
 int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
                           sfft_output_c *out);
 
+/* sfft_exec_sparse without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * per-call workspace (buckets, comb residues, votes, counts) is carved from a
+ * fixed 32 MiB device region of the library, one per GPU, whose address is
+ * looked up when a plan is built on that GPU; no hipMalloc, hipFree or
+ * allocating routine is called.  v1/v2 plans only; in, out->freq and
+ * out->val must be device memory (n values; out->capacity >=
+ * sfft_plan_output_capacity entries).  A plan whose workspace exceeds the
+ * region is rejected with an error before any RNG draw or launch.  Every
+ * call on one GPU uses the same region: calls that overlap on one GPU (from
+ * several threads or plans) race.  Returns 0 or -1. */
+int sfft_exec_sparse_noalloc(sfft_plan *plan, const complex_t *in, sfft_output_c *out);
+
 /* Reference-compatible synthetic input (src/simulation.cc:95-112,
  * src/timing_many.cc:117-139): k frequencies drawn with replacement as
  * floor(drand48() * n) with X[f] = 1, x = the unnormalised inverse DFT

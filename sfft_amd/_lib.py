@@ -21,6 +21,7 @@ EXPORTS = [
     "sfft_plan_last_status",
     "sfft_plan_set_schedule", "sfft_plan_get_schedule", "sfft_plan_autotune",
     "sfft_exec_many_plans", "sfft_plan_enable_graphs", "sfft_generate_input",
+    "sfft_exec_sparse_noalloc",
 ]
 
 # Every symbol include/sfft_server.h declares.
@@ -85,6 +86,7 @@ def load():
     L.sfft_plan_output_capacity.restype = C.c_int64
     L.sfft_exec_sparse.argtypes = [PP, P, C.POINTER(SfftOutput)]
     L.sfft_exec_many_sparse.argtypes = [PP, C.c_int, P, C.POINTER(SfftOutput)]
+    L.sfft_exec_sparse_noalloc.argtypes = [PP, P, C.POINTER(SfftOutput)]
     L.sfft_plan_set_precision.argtypes = [PP, C.c_int]
     L.sfft_plan_seed.argtypes = [PP, C.c_uint, C.c_long]
     L.sfft_plan_set_stream.argtypes = [PP, P]

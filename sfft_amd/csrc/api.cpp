@@ -265,6 +265,71 @@ int sfft_exec_many_sparse(sfft_plan *plan, int num, const complex_t *const *in,
   return run(P, num, in, nullptr, out);
 }
 
+int sfft_exec_sparse_noalloc(sfft_plan *plan, const complex_t *in, sfft_output_c *out)
+{
+  set_error("");
+  Plan *P = get(plan);
+  if (!P || !out) {
+    if (P) set_error("null output");
+    return -1;
+  }
+  if (P->version == SFFT_VERSION_3) {
+    set_error("sfft_exec_sparse_noalloc runs v1/v2 plans only");
+    return -1;
+  }
+  const int64_t cap = P->output_capacity();
+  {
+    PtrKinds kinds;  // fixed-size cache: no allocation
+    if (!kinds.covers(in, (size_t)P->n * 16) || !kinds.covers(out->freq, (size_t)cap * 4) ||
+        !kinds.covers(out->val, (size_t)cap * 16) || out->capacity < cap) {
+      set_error("sfft_exec_sparse_noalloc takes a device input of n values and device "
+                "outputs of sfft_plan_output_capacity entries");
+      return -1;
+    }
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("no current device");
+    return -1;
+  }
+  void *scr = k::noalloc_scratch(dev);
+  if (!scr) {
+    set_error("no scratch region on this device: build a plan on it first");
+    return -1;
+  }
+  auto lk = hold(P);
+  // the workspace of this call: views into the device scratch region and a
+  // thread-local host staging buffer (params, pointer tables, count)
+  alignas(256) static thread_local char host_stage[4096];
+  Arena arena;
+  arena.base = static_cast<char *>(scr);
+  arena.size = k::kNoAllocScratch;
+  Workspace ws;
+  ws.use_arena(&arena);
+  ws.pinned = host_stage;
+  ws.pinned_bytes = sizeof(host_stage);
+  ws.pinned_view = true;
+  const double2 *ins[1] = {reinterpret_cast<const double2 *>(in)};
+  int32_t *frs[1] = {out->freq};
+  double2 *vas[1] = {reinterpret_cast<double2 *>(out->val)};
+  int64_t count = 0;
+  BatchIO io;
+  io.S = 1;
+  io.in = ins;
+  io.freq = frs;
+  io.val = vas;
+  io.cap = out->capacity;
+  io.counts = &count;
+  io.fixed_ws = true;
+  Workspace *own = P->ws;
+  P->ws = &ws;
+  const int rc = exec_batch(P, io);
+  P->ws = own;
+  P->last_S = 0;  // the buckets of this call live in the scratch region
+  if (rc == 0) out->count = count;
+  return rc;
+}
+
 int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *const *in,
                          sfft_output_c *out)
 {

@@ -114,6 +114,19 @@ double Rng::next_drand48()
 bool DevBuf::ensure(size_t nbytes)
 {
   if (nbytes <= bytes && p) return true;
+  if (arena) {
+    // carved once per exec from the fixed region, 256-byte aligned
+    const size_t at = (arena->used + 255) & ~size_t(255);
+    if (p || at + nbytes > arena->size) {
+      set_error("allocation-free exec: the plan needs more than the " +
+                std::to_string(arena->size >> 20) + " MiB scratch region");
+      return false;
+    }
+    p = arena->base + at;
+    bytes = nbytes;
+    arena->used = at + nbytes;
+    return true;
+  }
   release();
   if (nbytes == 0) nbytes = 256;
   if (hipMalloc(&p, nbytes) != hipSuccess) {
@@ -128,7 +141,7 @@ bool DevBuf::ensure(size_t nbytes)
 
 void DevBuf::release()
 {
-  if (p) (void)hipFree(p);
+  if (p && !arena) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
 }
@@ -391,6 +404,12 @@ Plan *make_plan(int n, int k, sfft_version v)
     return nullptr;
   }
   P->own_stream = true;
+  // the allocation-free exec's scratch address on this device, looked up
+  // here (plan construction is its set-up) rather than in that exec
+  if (k::noalloc_setup() != 0) {
+    free_plan(P);
+    return nullptr;
+  }
   return P;
 }
 
@@ -404,13 +423,21 @@ size_t Workspace::device_bytes() const
   return b;
 }
 
+void Workspace::use_arena(Arena *a)
+{
+  DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
+                    &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
+                    &comb, &mag, &tmp64, &tmp32, &multi};
+  for (DevBuf *d : bufs) d->arena = a;
+}
+
 void Workspace::release()
 {
   DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
                     &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
                     &comb, &mag, &tmp64, &tmp32, &multi};
   for (DevBuf *d : bufs) d->release();
-  if (pinned) (void)hipHostFree(pinned);
+  if (pinned && !pinned_view) (void)hipHostFree(pinned);
   pinned = nullptr;
   pinned_bytes = 0;
 }
@@ -488,6 +515,10 @@ void free_plan(Plan *P)
 static int ensure_pinned(Plan *P, size_t bytes)
 {
   if (bytes <= P->ws->pinned_bytes) return 0;
+  if (P->ws->pinned_view) {
+    set_error("host staging of the allocation-free exec too small");
+    return -1;
+  }
   if (P->ws->pinned) (void)hipHostFree(P->ws->pinned);
   P->ws->pinned = nullptr;
   P->ws->pinned_bytes = 0;
@@ -757,20 +788,6 @@ int exec_batch(Plan *P, BatchIO &io)
   const void **t_dense = reinterpret_cast<const void **>(pin + param_bytes + 3 * tab_bytes);
   int64_t *h_counts = reinterpret_cast<int64_t *>(pin + param_bytes + 4 * tab_bytes);
 
-  // RNG draws in signal order (src/computefourier-1.0-2.0.cc:465-474, 62).
-  const int sigma = p.comb ? n / p.W_comb : 1;
-  for (int s = 0; s < S; s++) {
-    int *row = hp + (size_t)s * stride;
-    Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
-    for (int l = 0; l < L; l++) {
-      int a = 0;
-      while ((a & 1) == 0) a = (int)(R.next_random() % n);  // gcd(a, n) == 1, n = 2^p
-      row[l] = a;
-      row[L + l] = mod_inverse_pow2(a, n);
-    }
-    for (int c = 0; c < cl; c++)
-      row[2 * L + c] = (int)(unsigned)std::floor(R.next_drand48() * sigma);
-  }
   const int vchunks = p.comb ? 0 : k::vote_chunks(n, p.B_loc, num, p.loc_loops, p.thr_loops);
   // v1 keeps its location loops in FP64 under fp32 (selection-critical)
   const bool loc64 = fp32 && !p.comb;
@@ -807,6 +824,22 @@ int exec_batch(Plan *P, BatchIO &io)
       (big64 && !P->ws->tmp64.ensure((size_t)S * xs * 16)) ||
       (big32 && !P->ws->tmp32.ensure((size_t)S * xs * 8)))
     return -1;
+  // RNG draws in signal order (src/computefourier-1.0-2.0.cc:465-474, 62),
+  // made once every buffer is in place so that a call failing above
+  // consumes none.
+  const int sigma = p.comb ? n / p.W_comb : 1;
+  for (int s = 0; s < S; s++) {
+    int *row = hp + (size_t)s * stride;
+    Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
+    for (int l = 0; l < L; l++) {
+      int a = 0;
+      while ((a & 1) == 0) a = (int)(R.next_random() % n);  // gcd(a, n) == 1, n = 2^p
+      row[l] = a;
+      row[L + l] = mod_inverse_pow2(a, n);
+    }
+    for (int c = 0; c < cl; c++)
+      row[2 * L + c] = (int)(unsigned)std::floor(R.next_drand48() * sigma);
+  }
   hipStream_t st = P->stream;
   char *dtab = reinterpret_cast<char *>(P->ws->params.p) + param_bytes;
 
@@ -877,12 +910,16 @@ int exec_batch(Plan *P, BatchIO &io)
     a.stream_blocks = 2 * cus;  // 8 resident wavefronts per CU
   }
 
-  if (P->timing && !P->ev_made) {
+  // an arena workspace creates no runtime objects: no timing events, no
+  // second stream, no graphs
+  const bool timing = P->timing && !io.fixed_ws;
+  const bool two_streams = p.comb && !io.fixed_ws;
+  if (timing && !P->ev_made) {
     for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
     for (hipEvent_t &e : P->ev_comb) HIP_OK(hipEventCreate(&e));
     P->ev_made = true;
   }
-  if (p.comb && !P->stream2) {
+  if (two_streams && !P->stream2) {
     HIP_OK(hipStreamCreateWithFlags(&P->stream2, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
@@ -896,12 +933,15 @@ int exec_batch(Plan *P, BatchIO &io)
     };
     HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, param_bytes + 4 * tab_bytes,
                           hipMemcpyHostToDevice, st));
-    if (p.comb) HIP_OK(hipEventRecord(P->ev_fork, st));
+    if (two_streams) HIP_OK(hipEventRecord(P->ev_fork, st));
     mark(1);
     // the fold is enqueued before the comb: for a single signal the host's
     // launch calls are on the latency path, and the fold branch is longer
     if ((launches[1] = k::fold_fft(a, st, 0)) < 0) return -1;
-    if (p.comb) {
+    if (p.comb && !two_streams) {
+      if (k::comb(a, st) != 0) return -1;
+      launches[0] = cl > 1 ? 2 : 1;
+    } else if (p.comb) {
       hipStream_t st2 = P->stream2;
       HIP_OK(hipStreamWaitEvent(st2, P->ev_fork, 0));
       if (timed) (void)hipEventRecord(P->ev_comb[0], st2);
@@ -915,7 +955,7 @@ int exec_batch(Plan *P, BatchIO &io)
     mark(3);
     if (!p.comb && k::select_vote_v1(a, st) != 0) return -1;
     launches[3] = p.comb ? 0 : 4;
-    if (p.comb) HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
+    if (two_streams) HIP_OK(hipStreamWaitEvent(st, P->ev_join, 0));
     mark(4);
     if (k::estimate(a, st) != 0) return -1;
     launches[4] = 1;
@@ -925,7 +965,7 @@ int exec_batch(Plan *P, BatchIO &io)
   };
   P->last_fp32 = fp32;
   P->last_S = S;
-  if (P->use_graphs && !P->timing) {
+  if (P->use_graphs && !timing && !io.fixed_ws) {
     // replay the batch's stream work from a graph captured on the first
     // batch of this shape; the workspace and staging addresses are part of
     // the key (kernel arguments and copy nodes hold them)
@@ -960,11 +1000,11 @@ int exec_batch(Plan *P, BatchIO &io)
       P->graphs.push_back(Plan::GraphEntry{key, sig, ex});
     }
     HIP_OK(hipGraphLaunch(ex, st));
-  } else if (enqueue(P->timing) != 0) {
+  } else if (enqueue(timing) != 0) {
     return -1;
   }
   HIP_OK(hipStreamSynchronize(st));
-  if (P->timing) {
+  if (timing) {
     for (int i = 0; i < 5; i++) {
       // only stages that ran: an event never recorded fails the query, and
       // that error would be reported by the next launch check

@@ -138,9 +138,17 @@ int gcd_int(int a, int b);
 int mod_inverse_int(int a, int n);
 
 // ---------------------------------------------------------------- device
+// A fixed device region that DevBufs are carved from instead of allocated
+// (the allocation-free exec).
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+};
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  Arena *arena = nullptr;      // carve from here (never hipMalloc / hipFree)
   bool ensure(size_t nbytes);  // grow-only
   void release();
   template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
@@ -157,6 +165,9 @@ struct Workspace {
   DevBuf multi;  // descriptors of a multi-plan launch
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
+  bool pinned_view = false;  // pinned is the caller's host buffer (not freed)
+  // every buffer carved from one fixed region (nothing allocated or freed)
+  void use_arena(Arena *a);
   size_t device_bytes() const;
   void release();
 };
@@ -260,6 +271,9 @@ struct BatchIO {
   double2 *const *dense = nullptr;  // n values each, already zero-filled
   int64_t cap = 0;
   int64_t *counts = nullptr;        // host: S located counts (out)
+  // the plan's workspace is an arena: no graph capture, no timing events,
+  // no second stream (each would create runtime objects)
+  bool fixed_ws = false;
 };
 
 // Draws the RNG values of S signals in signal order, runs the kernels on
@@ -386,6 +400,14 @@ int radix2_fft64(double2 *data, double2 *tmp, int N, int64_t nb_outer, int nb_in
                  int64_t si, const double2 *tw, int tw_nmax, hipStream_t st);
 int radix2_fft32(float2 *data, float2 *tmp, int N, int64_t nb_outer, int nb_inner, int64_t so,
                  int64_t si, const float2 *tw, int tw_nmax, hipStream_t st);
+
+// The allocation-free exec's scratch region (kNoAllocScratch bytes of
+// device memory per GPU, a file-scope __device__ array): noalloc_setup()
+// looks up its address on the current device (once per device; a plan's
+// construction calls it), noalloc_scratch() returns it or null.
+constexpr size_t kNoAllocScratch = (size_t)32 << 20;
+int noalloc_setup();
+void *noalloc_scratch(int dev);
 
 int comb(const V12Launch &a, hipStream_t st);
 // part 0: the fold (fused with the bucket FFTs when fold_mode == 0);

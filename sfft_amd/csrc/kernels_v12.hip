@@ -15,6 +15,8 @@
 // (signal, chunk).  All index arithmetic is exact: for n = 2^p the low 32
 // bits of i * a already give (i * a) mod n.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -1109,6 +1111,49 @@ int launch_failed(const char *what)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- scratch
+// Device memory of the allocation-free exec (sfft_exec_sparse_noalloc): the
+// workspace of one signal is carved from it instead of allocated.  Every GPU
+// the library runs on holds its own copy for the life of the process.  In
+// host code `scratch` is only a stand-in for the symbol: its device address
+// comes from hipGetSymbolAddress, once per device, and only that address is
+// passed to kernels and copies.
+static __device__ __align__(256) char scratch[kNoAllocScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_setup()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("noalloc set-up: no current device");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[dev]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(std::string("noalloc set-up: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    return -1;
+  }
+  g_scratch[dev] = ptr;
+  g_scratch_done[dev] = true;
+  return 0;
+}
+
+void *noalloc_scratch(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[dev];
+}
 
 // Below this many comb transforms per batch they run with kCombWide threads.
 constexpr int kFewCombs = 64, kCombWide = 1024;
