@@ -56,10 +56,9 @@ def parse():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the v1 / v2 single-signal and batched v3 configurations")
     ap.add_argument("--serve-requests", type=int, default=160)
-    ap.add_argument("--serve-rate", type=float, default=100.0,
-                    help="Poisson arrival rate of the serving run, requests/s")
-    ap.add_argument("--serve-rate-high", type=float, default=2000.0,
-                    help="a second FP32+refine serving run at this rate (0: none)")
+    ap.add_argument("--serve-rates", default="200,2000,0",
+                    help="Poisson arrival rates of the serving runs, requests/s "
+                         "(0: every request at once, the saturated throughput)")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=16)
     ap.add_argument("--only-configs", action="store_true",
@@ -73,21 +72,25 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_PROFILE = "profiles/traffic_fold_chunks_v2_n22_k64_b256.md"
+
+
 def measured_traffic(kernel, S, n, k):
     """HBM bytes per launch of the fold from the committed rocprofv3 counter
-    runs (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), when this run
-    is the profiled workload; else None."""
-    if kernel != "k_fold_stream<true>" or (S, n, k) != (256, 1 << 22, 64):
-        return None
-    path = os.path.join(ROOT, "profiles", "traffic_v2_n22_k64_b256_fp32.md")
+    runs of this kernel on this workload (FETCH_SIZE and WRITE_SIZE, scaled
+    by a 1 GiB copy), and the file they come from; (None, None) for another
+    kernel or workload."""
+    if (S, n, k) != (256, 1 << 22, 64):
+        return None, None
+    key = "traffic_bytes_per_launch %s 256x2^22 k64:" % kernel
     try:
-        with open(path) as fh:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch k_fold_stream 256x2^22 k64:"):
-                    return float(line.split(":")[1])
+                if line.startswith(key):
+                    return float(line.split(":")[1]), TRAFFIC_PROFILE
     except OSError:
         pass
-    return None
+    return None, None
 
 
 def copy_rate(torch, dev):
@@ -106,6 +109,11 @@ def copy_rate(torch, dev):
     del src, dst
     torch.cuda.empty_cache()
     return gbs
+
+
+def log(msg):
+    """Progress of the run on stderr (the JSON result is the only stdout line)."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
 
 
 def pct(a, q):
@@ -145,6 +153,7 @@ def main():
     torch.cuda.synchronize()
 
     # ---------------------------------------------------------- headline
+    log("signals ready; headline plan")
     p32 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp32_refine", seed=(17, 4242))
     if args.schedule:  # fold schedule only; same results
         p32.set_schedule(*[int(v) for v in args.schedule.split(",")])
@@ -153,8 +162,24 @@ def main():
         schedule32 = p32.autotune(batch=min(S, 64))
     cap = p32.capacity
     fr32, va32 = p32.make_sparse_outputs(S)
+    # headline: the batch replayed from a captured HIP graph (uploads, comb
+    # beside the fold on a second stream, fold, bucket FFTs, estimation,
+    # count read-back); the first exec of the shape captures it
+    p32.enable_graphs(True)
     for _ in range(args.warmup):
         p32.execute_many_sparse(x, fr32, va32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        counts32 = p32.execute_many_sparse(x, fr32, va32)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    p32.enable_graphs(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    tps32 = S * args.steps / (t1 - t0)
+    # the same batch launched eagerly, with HIP events on the library's
+    # streams around each stage (the roofline's kernel times)
+    p32.execute_many_sparse(x, fr32, va32)
     torch.cuda.synchronize()
     p32.enable_timing(True)
     t0 = time.perf_counter()
@@ -164,17 +189,19 @@ def main():
     t1 = time.perf_counter()
     stages32 = p32.stage_times()
     p32.enable_timing(False)
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    tps32 = S * args.steps / (t1 - t0)
+    tps32_eager = S * args.steps / (t1 - t0)
 
     result = {
         "metric": METRIC, "value": tps32, "unit": "transforms/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64", "data": "synthetic",
-        "config": {"workload": "sFFT v2 batched exec_many_sparse, FP32 buckets + FP64 "
-                               "estimation, %d signals, n=2^%d, k=%d, SNR %g dB"
-                               % (S, int(np.log2(n)), k, args.snr),
+        "eager": {"value": tps32_eager, "unit": "transforms/s",
+                  "ms_per_step": S / tps32_eager * 1e3,
+                  "form": "the same batch launched eagerly (stage events on)"},
+        "config": {"workload": "sFFT v2 batched exec_many_sparse replayed from a HIP graph, "
+                               "FP32 buckets + FP64 estimation, %d signals, n=2^%d, k=%d, "
+                               "SNR %g dB" % (S, int(np.log2(n)), k, args.snr),
                    "n": n, "k": k, "batch": S, "version": 2, "snr_db": args.snr,
                    "params": dict(zip(["B_loc", "B_est", "B_thresh", "W_comb",
                                        "comb_loops", "loc_loops", "est_loops",
@@ -193,17 +220,18 @@ def main():
     fold_ms, fold_launches = stages32["fold"]
     fold_avg_ms = fold_ms / max(1, fold_launches)
     achieved = fold_bytes / (fold_avg_ms * 1e-3) / 1e9
-    kname = "k_fold_fft32" if schedule32[0] == 0 else "k_fold_stream<true>"
-    traffic = measured_traffic(kname, S, n, k)
+    kname = "k_fold_chunks"
+    traffic, traffic_src = measured_traffic(kname, S, n, k)
     result["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": traffic,
+                          "traffic": traffic, "traffic_source": traffic_src,
                           "alg_bytes_per_launch": fold_bytes,
                           "avg_launch_ms": fold_avg_ms, "launches": fold_launches}
     if traffic:  # line traffic the counters show, over this run's launch time
         result["roofline"]["traffic_gbs"] = traffic / (fold_avg_ms * 1e-3) / 1e9
         result["roofline"]["traffic_frac"] = result["roofline"]["traffic_gbs"] / HBM_PEAK_GBS
     result["roofline"]["copy_gbs"] = copy_rate(torch, dev)
+    log("headline %.0f transforms/s (graph), %.0f eager" % (tps32, tps32_eager))
     result["stages_ms_per_step"] = {s: v[0] / args.steps for s, v in stages32.items() if v[1]}
 
     if not args.only_fp32:
@@ -228,6 +256,7 @@ def main():
                                 "form": "fp32+refine, 1 signal, replayed HIP graph",
                                 "eager_p50": pct(lat_eager, 50), "eager_p99": pct(lat_eager, 99)}
 
+        log("latency done")
         # ------------------------------------------------------ FP64 forms
         p64 = sfft_amd.sfft(n, k, 2, strict=False, precision="fp64", seed=(17, 4242))
         p64.autotune(batch=min(S, 64))
@@ -287,6 +316,7 @@ def main():
         result["speedup_fp32_vs_fp64_plain"] = tps32 / tps_plain
         result["speedup_fp32_vs_fp64_batched"] = tps32 / tps64b
 
+        log("fp64 forms done")
         # ------------------------------------------------------ accuracy
         # FP64 full FFT of the signal as the oracle; the reference's v1/v2
         # values are conj(X) (src/computefourier-1.0-2.0.cc:383-398), so
@@ -324,15 +354,18 @@ def main():
         if "fr64" in locals():
             del fr64, va64
         torch.cuda.empty_cache()
+        log("other configurations")
         result["configs"] = other_configs(torch, dev)
 
     # ---------------------------------------------------------- serving
     if not args.no_serve and not args.only_fp32:
         torch.cuda.empty_cache()
+        log("serving")
         result["serving"] = serving(args, torch, dev)
 
     # ---------------------------------------------------------- CPU
     if not args.no_cpu and not args.only_fp32:
+        log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(args, locals().get("gpu_check"))
     print(json.dumps(result), flush=True)
 
@@ -403,7 +436,29 @@ def other_configs(torch, dev):
         counts = execute_many_plans(order, xs, outs, trim=False)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
-    res = [(fr[:int(c)], va[:int(c)]) for (fr, va), c in zip(outs, counts)]
+    # oracle parity of the first entry of each plan: the plans' first draws
+    # again (re-seeded), the oracle on the plans' filters
+    import oracle
+    for kk in ks:
+        plans[kk].seed(17 + kk, 4242)
+    counts0 = execute_many_plans(order, xs, outs, trim=False)
+    torch.cuda.synchronize()
+    parity = {}
+    for j, kk in enumerate(ks):
+        t1, r1 = plans[kk].filter(0)
+        t2, r2 = plans[kk].filter(1)
+        P = oracle.V3(n, kk, bare=True)
+        P.set_filters(t1, r1, t2, r2)
+        oracle.seed_glibc(17 + kk, 4242)
+        ref = P.exec(xs[j].cpu().numpy(), P.draw())
+        c = int(counts0[j])
+        fr, va = outs[j][0][:c].cpu().numpy(), outs[j][1][:c].cpu().numpy()
+        same = (not ref["early"] and np.array_equal(fr, ref["freqs"])) or (ref["early"] and c == 0)
+        parity[str(kk)] = {"located_set_equal": bool(same),
+                           "value_maxabs_vs_oracle": float(np.abs(va - ref["vals"]).max())
+                           if same and c else 0.0, "located": c}
+        del P, t1, r1, t2, r2
+    res = [(fr[:int(c)], va[:int(c)]) for (fr, va), c in zip(outs, counts0)]
     found, err = [], 0.0
     for (fr, va), (fs, vs) in zip(res, truth):
         got = dict(zip(fr.cpu().numpy().tolist(), va.cpu().numpy().tolist()))
@@ -413,7 +468,8 @@ def other_configs(torch, dev):
     out["v3_n24_b64_k200_500_1000"] = {
         "transforms_per_s": S / dt, "ms_per_batch": dt * 1e3, "form": "fp64, one launch",
         "recovered_fraction_by_k": {str(kk): float(np.mean(found[j::3])) for j, kk in enumerate(ks)},
-        "max_value_error_recovered": err}
+        "max_value_error_recovered": err,
+        "oracle_parity_first_entry_by_k": parity}
     return out
 
 
@@ -422,12 +478,15 @@ def serving(args, torch, dev):
     {2^18, 2^20, 2^22, 2^24}, k = floor(10 * 200^U), v2 (SNR 30 dB) or v3
     (noiseless) 50/50; v2 draws the plan cannot run (W_Comb < 2k+1) are
     rejected and counted.  Inputs and plans are device-resident before the
-    arrivals start; the same trace runs in the FP32+refine and the FP64
-    forms, and once more in FP32+refine compressed to a higher rate (the
-    server then batches and overlaps requests).  Latency = submit to the
-    end of the request's batch."""
+    arrivals start.  The same trace runs at several arrival rates (rate 0:
+    every request submitted at once, the saturated throughput) in the
+    FP32+refine and the FP64 forms, each form on one server whose plans stay
+    resident across the loads.  Latency = submit to the end of the request's
+    batch.  A sample of the results is compared with the same requests run
+    alone (exact equality)."""
     from concurrent.futures import ThreadPoolExecutor
 
+    import sfft_amd
     from sfft_amd import SfftError
     from sfft_amd.server import Server, output_capacity
     from sfft_amd.signals import signals_device
@@ -441,7 +500,7 @@ def serving(args, torch, dev):
         k = int(np.floor(10 * 200 ** rng.random()))
         v = 2 if rng.random() < 0.5 else 3
         reqs.append((n, k, v))
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, N))
+    unit = np.cumsum(rng.exponential(1.0, N))  # arrival times at rate 1
     caps = []
     for (n, k, v) in reqs:
         try:
@@ -463,45 +522,64 @@ def serving(args, torch, dev):
                      torch.empty(caps[i], dtype=torch.complex128, device=dev)))
     torch.cuda.synchronize()
     keys = sorted({reqs[i] for i in range(N) if caps[i] is not None})
-    out = {"requests": N, "rate_per_s": args.serve_rate, "workers": args.serve_workers,
-           "max_batch": args.serve_batch, "distinct_plans": len(keys),
-           "rejected_infeasible": sum(c is None for c in caps),
+    feasible = [i for i in range(N) if caps[i] is not None]
+    checked = feasible[::max(1, len(feasible) // 8)][:8]
+    out = {"requests": N, "workers": args.serve_workers, "max_batch": args.serve_batch,
+           "distinct_plans": len(keys), "rejected_infeasible": sum(c is None for c in caps),
            "mix": "n uniform {2^18,2^20,2^22,2^24}, k=floor(10*200^U), v2 SNR 30 dB / v3 "
                   "noiseless 50/50"}
-    runs = [("fp32_refine", args.serve_rate), ("fp64", args.serve_rate)]
-    if args.serve_rate_high > 0:
-        runs.append(("fp32_refine", args.serve_rate_high))
-    for prec, rate in runs:
-        name = prec if rate == args.serve_rate else "%s_at_%g_per_s" % (prec, rate)
-        at = arrivals * (args.serve_rate / rate)
+    loads = [float(r) for r in args.serve_rates.split(",")]
+    for prec in ("fp32_refine", "fp64"):
+        res = {}
         with Server(workers=args.serve_workers, max_batch=args.serve_batch) as srv:
             t0 = time.perf_counter()
             with ThreadPoolExecutor(8) as ex:
                 list(ex.map(lambda key: srv.prepare(key[0], key[1], key[2], prec), keys))
             prep_s = time.perf_counter() - t0
-            tickets, rejected = [], 0
-            start = time.perf_counter()
-            for i, (n, k, v) in enumerate(reqs):
-                dt = at[i] - (time.perf_counter() - start)
-                if dt > 0:
-                    time.sleep(dt)
-                try:
-                    tickets.append(srv.submit(xs[i], k, v, prec, (i + 1, 7 * i + 3),
-                                              outputs=outs[i]))
-                except SfftError:
-                    rejected += 1
-            lat = [srv.wait(t)[2] for t in tickets]
-            span = time.perf_counter() - start
+            for rate in loads:
+                s0 = srv.stats()
+                at = unit / rate if rate > 0 else np.zeros(N)
+                tickets, rejected = [], 0
+                start = time.perf_counter()
+                for i, (n, k, v) in enumerate(reqs):
+                    dt = at[i] - (time.perf_counter() - start)
+                    if dt > 0:
+                        time.sleep(dt)
+                    try:
+                        tickets.append((i, srv.submit(xs[i], k, v, prec, (i + 1, 7 * i + 3),
+                                                      outputs=outs[i])))
+                    except SfftError:
+                        rejected += 1
+                got = {i: srv.wait(t) for i, t in tickets}
+                span = time.perf_counter() - start
+                s1 = srv.stats()
+                lat = [g[2] for g in got.values()]
+                done = s1["completed"] - s0["completed"]
+                nb = s1["batches"] - s0["batches"]
+                res["saturated" if rate <= 0 else "%g_per_s" % rate] = {
+                    "offered_rate_per_s": rate if rate > 0 else None,
+                    "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                    "mean_ms": float(np.mean(lat)), "completed": done,
+                    "rejected": rejected, "failed": s1["failed"] - s0["failed"],
+                    "transforms_per_s": done / span, "batches": nb,
+                    "mean_batch": (s1["batched_requests"] - s0["batched_requests"]) / max(1, nb),
+                    "max_concurrent_batches": s1["max_concurrent_batches"]}
             st = srv.stats()
-        out[name] = {"rate_per_s": rate, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                     "mean_ms": float(np.mean(lat)), "completed": st["completed"],
-                     "rejected": rejected, "failed": st["failed"],
-                     "transforms_per_s": st["completed"] / span,
-                     "batches": st["batches"],
-                     "mean_batch": st["batched_requests"] / max(1, st["batches"]),
-                     "max_concurrent_batches": st["max_concurrent_batches"],
+        # the last load's results of a sample of requests against each run alone
+        mism = 0
+        for i in checked:
+            n, k, v = reqs[i]
+            fr, va, _ = got[i]
+            plan = sfft_amd.sfft(n, k, v, strict=False, precision=prec, seed=(i + 1, 7 * i + 3))
+            f1, v1 = plan.make_sparse_outputs(1)
+            c1 = int(plan.execute_many_sparse([xs[i]], f1, v1)[0])
+            torch.cuda.synchronize()
+            mism += not (c1 == len(fr) and torch.equal(f1[0, :c1], fr) and
+                         torch.equal(v1[0, :c1], va))
+            del plan, f1, v1
+        out[prec] = {"loads": res, "plan_prepare_s": prep_s,
                      "hbm_peak_gb": st["hbm_peak"] / 1e9,
-                     "plan_prepare_s": prep_s}
+                     "checked_vs_alone": len(checked), "mismatches_vs_alone": mism}
     return out
 
 
@@ -549,15 +627,29 @@ def cpu_baseline(args, check=None):
         a[i], ai[i], off[i] = P.draw()
     xp = np.array([xs[i % distinct].ctypes.data for i in range(num)], dtype=np.uint64)
     op = np.array([o.ctypes.data for o in outs], dtype=np.uint64)
+    # the timing build (the reference's -O3 -ffast-math flags) is the
+    # baseline; the parity build (-O2, no contraction) is timed beside it
+    t0 = time.perf_counter()
+    P.exec_many(xp, a, ai, off, op, -threads, fast=True)
+    dt = time.perf_counter() - t0
     t0 = time.perf_counter()
     P.exec_many(xp, a, ai, off, op, -threads)
-    dt = time.perf_counter() - t0
+    dt_strict = time.perf_counter() - t0
     # 1-thread latency: sequential single-signal execs (first 20 draws)
     lat1 = []
     for i in range(min(20, num)):
         ts = time.perf_counter()
-        P.exec_many(xp[i:i + 1], a[i:i + 1], ai[i:i + 1], off[i:i + 1], op[:1], -1)
+        P.exec_many(xp[i:i + 1], a[i:i + 1], ai[i:i + 1], off[i:i + 1], op[:1], -1, fast=True)
         lat1.append((time.perf_counter() - ts) * 1e3)
+    # FFTW (the reference's FFT library) is not in this image: the
+    # restatement uses its own FFT either way; say whether the host has it
+    try:
+        import ctypes.util
+        fftw = ctypes.util.find_library("fftw3") or ""
+        C.CDLL("libfftw3.so.3")
+        fftw = fftw or "libfftw3.so.3"
+    except OSError:
+        fftw = ""
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -572,6 +664,13 @@ def cpu_baseline(args, check=None):
                       "reference semantics incl. the dense zero-filled output; "
                       "OpenMP over signals; %.1f s wall"
                       % (num, distinct, int(np.log2(n)), k, args.snr, dt),
+            "build": "oracle/_build/libsfft_oracle_fast.so: -O3 -ffast-math -march=x86-64-v3 "
+                     "-fopenmp (the reference's wscript:10 flags, x86-64-v3 for native)",
+            "parity_build_value": num / dt_strict,
+            "parity_build": "-O2 -ffp-contract=off (the bit-exact checker)",
+            "fft": "own radix-2/Bluestein FP64 FFT" + (
+                "; %s present on this host but not used" % fftw if fftw else
+                "; libfftw3.so.3 not found on this host"),
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model,
             "latency_1thread_ms": {"p50": pct(lat1, 50), "p99": pct(lat1, 99),
                                    "samples": len(lat1)}}

@@ -241,6 +241,17 @@ int sfft_plan_stage_times(const sfft_plan *plan, double *ms6,
 int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
                            int64_t count);
 
+/* Diagnostics of the v2 location votes (src/computefourier-1.0-2.0.cc:
+ * 125-184), which never change the output (every frequency they append is
+ * already in the comb prefill, cf12 505-512) and are skipped by the exec.
+ * With enabled != 0, later FP64 execs of this v2 plan also select the top
+ * buckets of the location loops and count the votes on the GPU.
+ * sfft_debug_comb_votes then writes the frequencies the reference appends to
+ * its hit list, in the reference's order, for signal `signal` of the last
+ * batch (at most cap of them) and returns how many there are, or -1. */
+int sfft_plan_debug_comb_votes(sfft_plan *plan, int enabled);
+int64_t sfft_debug_comb_votes(const sfft_plan *plan, int signal, int32_t *hits, int64_t cap);
+
 /* The plan's filters (which: 0 = location / first Gauss filter, 1 =
  * estimation / second Gauss filter): its w taps into time (capacity
  * time_cap) and its response at offsets -half..half into resp (capacity

@@ -102,6 +102,8 @@ def lib():
         L.or_v3_params_get.argtypes = [C.c_int, C.c_int, C.POINTER(V3Params)]
         L.or_v3_plan_make.argtypes = [C.c_int, C.c_int]
         L.or_v3_plan_make.restype = C.POINTER(V3Plan)
+        L.or_v3_plan_make_bare.argtypes = [C.c_int, C.c_int]
+        L.or_v3_plan_make_bare.restype = C.POINTER(V3Plan)
         L.or_v3_plan_free.argtypes = [C.POINTER(V3Plan)]
         L.or_v3_plan_set_filters.argtypes = [C.POINTER(V3Plan), P, P, P, P]
         L.or_v3_draw.argtypes = [C.c_int, P]
@@ -112,6 +114,25 @@ def lib():
         L.fm_fft32.argtypes = [P, C.c_int, C.c_int]
         _lib = L
     return _lib
+
+
+FAST_PATH = os.path.join(HERE, "_build", "libsfft_oracle_fast.so")
+_lib_fast = None
+
+
+def lib_fast():
+    """The timing build of the oracle (the reference's -O3 -ffast-math
+    flags; not for parity): or_v12_exec_many only.  Plans made by lib()
+    run on it (same source, same structures)."""
+    global _lib_fast
+    if _lib_fast is None:
+        if not os.path.exists(FAST_PATH):
+            build(with_ref=False)
+        L = C.CDLL(FAST_PATH)
+        P = C.c_void_p
+        L.or_v12_exec_many.argtypes = [C.POINTER(V12Plan), C.c_int, P, P, P, P, P, C.c_int]
+        _lib_fast = L
+    return _lib_fast
 
 
 def ref_helpers():
@@ -248,17 +269,20 @@ class V12:
                "hits_found": t.hits_found, "hits_prefill": t.hits_prefill}
         return res
 
-    def exec_many(self, xs_ptrs, draws_a, draws_ai, draws_off, outs_ptrs, threads):
-        """Timed CPU path: OpenMP over signals with pre-drawn RNG values."""
-        lib().or_v12_exec_many(self._h, len(xs_ptrs), _p(xs_ptrs), _p(draws_a),
-                               _p(draws_ai), _p(draws_off), _p(outs_ptrs), threads)
+    def exec_many(self, xs_ptrs, draws_a, draws_ai, draws_off, outs_ptrs, threads, fast=False):
+        """Timed CPU path: OpenMP over signals with pre-drawn RNG values;
+        fast=True runs the timing build (the reference's compiler flags)."""
+        L = lib_fast() if fast else lib()
+        L.or_v12_exec_many(self._h, len(xs_ptrs), _p(xs_ptrs), _p(draws_a),
+                           _p(draws_ai), _p(draws_off), _p(outs_ptrs), threads)
 
 
 class V3:
-    """Oracle plan for sFFT v3."""
+    """Oracle plan for sFFT v3 (bare=True: no filter design, for
+    set_filters)."""
 
-    def __init__(self, n, k):
-        self._h = lib().or_v3_plan_make(n, k)
+    def __init__(self, n, k, bare=False):
+        self._h = (lib().or_v3_plan_make_bare if bare else lib().or_v3_plan_make)(n, k)
         self.p = self._h.contents.p
         self.n = self.p.n
 

@@ -792,6 +792,20 @@ or_v3_plan *or_v3_plan_make(int n, int k)
   return P;
 }
 
+/* The same plan with its filters left for or_v3_plan_set_filters (no
+ * filter design: the n-point transforms take seconds at n = 2^24). */
+or_v3_plan *or_v3_plan_make_bare(int n, int k)
+{
+  or_v3_plan *P = (or_v3_plan *)calloc(1, sizeof(*P));
+  or_v3_params_get(n, k, &P->p);
+  int N = P->p.n;
+  P->time1 = (or_cplx *)calloc((size_t)P->p.w_g1, sizeof(or_cplx));
+  P->freq1 = (or_cplx *)calloc((size_t)N, sizeof(or_cplx));
+  P->time2 = (or_cplx *)calloc((size_t)P->p.w_g2, sizeof(or_cplx));
+  P->freq2 = (or_cplx *)calloc((size_t)N, sizeof(or_cplx));
+  return P;
+}
+
 void or_v3_plan_set_filters(or_v3_plan *P, const or_cplx *time1,
                             const or_cplx *freq1, const or_cplx *time2,
                             const or_cplx *freq2)

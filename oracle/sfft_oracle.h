@@ -125,6 +125,7 @@ typedef struct {
 } or_v3_plan;
 
 or_v3_plan *or_v3_plan_make(int n, int k);
+or_v3_plan *or_v3_plan_make_bare(int n, int k);  /* filters from set_filters */
 void or_v3_plan_free(or_v3_plan *P);
 /* Replace the plan's two filters (w_g1/w_g2 taps, n-point responses). */
 void or_v3_plan_set_filters(or_v3_plan *P, const or_cplx *time1,

@@ -21,7 +21,7 @@ EXPORTS = [
     "sfft_plan_last_status",
     "sfft_plan_set_schedule", "sfft_plan_get_schedule", "sfft_plan_autotune",
     "sfft_exec_many_plans", "sfft_plan_enable_graphs", "sfft_generate_input",
-    "sfft_exec_sparse_noalloc",
+    "sfft_exec_sparse_noalloc", "sfft_plan_debug_comb_votes", "sfft_debug_comb_votes",
 ]
 
 # Every symbol include/sfft_server.h declares.
@@ -87,6 +87,9 @@ def load():
     L.sfft_exec_sparse.argtypes = [PP, P, C.POINTER(SfftOutput)]
     L.sfft_exec_many_sparse.argtypes = [PP, C.c_int, P, C.POINTER(SfftOutput)]
     L.sfft_exec_sparse_noalloc.argtypes = [PP, P, C.POINTER(SfftOutput)]
+    L.sfft_plan_debug_comb_votes.argtypes = [PP, C.c_int]
+    L.sfft_debug_comb_votes.argtypes = [PP, C.c_int, P, C.c_int64]
+    L.sfft_debug_comb_votes.restype = C.c_int64
     L.sfft_plan_set_precision.argtypes = [PP, C.c_int]
     L.sfft_plan_seed.argtypes = [PP, C.c_uint, C.c_long]
     L.sfft_plan_set_stream.argtypes = [PP, P]

@@ -591,6 +591,43 @@ int sfft_plan_stage_times(const sfft_plan *plan, double *ms6, int64_t *launches6
   return 0;
 }
 
+int sfft_plan_debug_comb_votes(sfft_plan *plan, int enabled)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if (P->version != SFFT_VERSION_2) {
+    set_error("comb votes exist in v2 plans only");
+    return -1;
+  }
+  auto lk = hold(P);
+  P->debug_votes = enabled != 0;
+  return 0;
+}
+
+int64_t sfft_debug_comb_votes(const sfft_plan *plan, int signal, int32_t *hits, int64_t cap)
+{
+  Plan *P = get(plan);
+  if (!P || signal < 0 || (!hits && cap > 0)) return -1;
+  auto lk = hold(P);
+  if (signal >= P->last_S || P->votes_stride == 0) {
+    set_error("no comb votes recorded for this signal (enable them on a v2 FP64 plan first)");
+    return -1;
+  }
+  const int64_t stride = P->votes_stride;
+  std::vector<int64_t> kv((size_t)stride);
+  if (hipMemcpy(kv.data(), P->ws->votes.as<int64_t>() + (size_t)signal * stride, stride * 8,
+                hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("vote copy failed");
+    return -1;
+  }
+  std::vector<std::pair<int64_t, int32_t>> app;
+  for (int64_t i = 0; i < stride / 2; i++)
+    if (kv[2 * i] >= 0) app.push_back({kv[2 * i], (int32_t)kv[2 * i + 1]});
+  std::sort(app.begin(), app.end());
+  for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)app.size()); i++) hits[i] = app[i].second;
+  return (int64_t)app.size();
+}
+
 int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
                            int64_t count)
 {

@@ -417,7 +417,7 @@ size_t Workspace::device_bytes() const
 {
   const DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
                           &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
-                          &comb, &mag, &tmp64, &tmp32, &multi};
+                          &comb, &mag, &tmp64, &tmp32, &multi, &votes};
   size_t b = 0;
   for (const DevBuf *d : bufs) b += d->bytes;
   return b;
@@ -427,7 +427,7 @@ void Workspace::use_arena(Arena *a)
 {
   DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
                     &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
-                    &comb, &mag, &tmp64, &tmp32, &multi};
+                    &comb, &mag, &tmp64, &tmp32, &multi, &votes};
   for (DevBuf *d : bufs) d->arena = a;
 }
 
@@ -435,7 +435,7 @@ void Workspace::release()
 {
   DevBuf *bufs[] = {&params, &xsamp, &xsamp32, &approved, &numcomb, &J, &hits,
                     &ptrs, &out_freq, &out_val, &count, &stage, &v3, &v3_status,
-                    &comb, &mag, &tmp64, &tmp32, &multi};
+                    &comb, &mag, &tmp64, &tmp32, &multi, &votes};
   for (DevBuf *d : bufs) d->release();
   if (pinned && !pinned_view) (void)hipHostFree(pinned);
   pinned = nullptr;
@@ -811,9 +811,12 @@ int exec_batch(Plan *P, BatchIO &io)
       !P->ws->hits.ensure((size_t)S * (p.comb ? 1 : cap) * 4) ||
       !P->ws->count.ensure((size_t)S * 8 + (p.comb ? 0 : (size_t)S * vchunks * 4)))
     return -1;
+  // v2 comb-vote diagnostic (FP64 buckets in natural order)
+  const bool dbg_votes = P->debug_votes && p.comb && !fp32;
+  if (dbg_votes && !P->ws->votes.ensure((size_t)S * 2 * cap * 8)) return -1;
   // scratch for the sizes above the LDS limits
   const bool big_comb = p.comb && p.W_comb > k::kMaxLdsComb;
-  const bool big_sel = !p.comb && p.B_loc > k::kMaxLdsSelect;
+  const bool big_sel = (!p.comb || dbg_votes) && p.B_loc > k::kMaxLdsSelect;
   const bool big64 = need64 && std::max(p.B_loc, p.B_est) > kMaxLdsFft64;
   const bool big32 = fp32 && std::max(p.B_loc, p.B_est) > 2 * kMaxLdsFft64;
   const size_t comb_elems = big_comb ? (size_t)S * cl * p.W_comb : 0;
@@ -898,7 +901,7 @@ int exec_batch(Plan *P, BatchIO &io)
                                                           : P->fold_mode;
   // alias-row bucket layout (schedule 1): v2, W a multiple of n/B for both
   // loop kinds, at least one wavefront of alias rows, FFTs in LDS
-  a.rows_layout = p.comb && a.fold_mode == 1 && p.W_comb % (n / p.B_loc) == 0 &&
+  a.rows_layout = p.comb && a.fold_mode == 1 && !dbg_votes && p.W_comb % (n / p.B_loc) == 0 &&
                   p.W_comb % (n / p.B_est) == 0 && n / p.W_comb >= 64 &&
                   std::max(p.B_loc, p.B_est) <= (fp32 ? 2 : 1) * kMaxLdsFft64;
   P->last_rows_layout = a.rows_layout;
@@ -959,12 +962,15 @@ int exec_batch(Plan *P, BatchIO &io)
     mark(4);
     if (k::estimate(a, st) != 0) return -1;
     launches[4] = 1;
+    if (dbg_votes && k::comb_vote_keys(a, P->ws->votes.as<int64_t>(), 2 * cap, st) != 0)
+      return -1;
     mark(5);
     HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
     return 0;
   };
   P->last_fp32 = fp32;
   P->last_S = S;
+  P->votes_stride = dbg_votes ? 2 * cap : 0;
   if (P->use_graphs && !timing && !io.fixed_ws) {
     // replay the batch's stream work from a graph captured on the first
     // batch of this shape; the workspace and staging addresses are part of

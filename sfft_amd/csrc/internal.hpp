@@ -163,6 +163,7 @@ struct Workspace {
   // bucket FFT passes)
   DevBuf comb, mag, tmp64, tmp32;
   DevBuf multi;  // descriptors of a multi-plan launch
+  DevBuf votes;  // v2 comb-vote diagnostic: S x 2 x located int64
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
   bool pinned_view = false;  // pinned is the caller's host buffer (not freed)
@@ -186,6 +187,8 @@ struct Plan {
   bool last_fp32 = false;  // precision of the buckets in ws_xsamp
   bool last_rows_layout = false;  // the last batch stored buckets by alias row
   bool ref_conj = true;
+  bool debug_votes = false;  // v2 FP64: count the comb votes (diagnostic)
+  int64_t votes_stride = 0;  // int64 per signal in ws->votes of the last batch
   Rng rng;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -414,6 +417,11 @@ int comb(const V12Launch &a, hipStream_t st);
 // part 1: the bucket FFTs (nothing when fused).  Returns launches or -1.
 int fold_fft(const V12Launch &a, hipStream_t st, int part);
 int select_vote_v1(const V12Launch &a, hipStream_t st);
+// Diagnostic of v2 plans in FP64 with buckets in natural order: the
+// location-loop selections and, per located frequency (prefill order), the
+// pair (append key, frequency) of the reference's comb votes, key -1 for a
+// frequency never appended; keys holds S x stride int64.
+int comb_vote_keys(const V12Launch &a, int64_t *keys, int64_t stride, hipStream_t st);
 int estimate(const V12Launch &a, hipStream_t st);
 int zero_dense(double2 *out, int64_t n, hipStream_t st);
 // total values of uniform noise in [-1, 1)^2 (synthetic tuning input)

@@ -556,6 +556,65 @@ __global__ void __launch_bounds__(kBlock) k_vote_write(V12Launch a, int chunks)
   }
 }
 
+// ---------------------------------------------------------------- v2 votes (diagnostic)
+// The v2 location votes (src/computefourier-1.0-2.0.cc:125-184), which never
+// change the output (every frequency they append is already in the comb
+// prefill), counted for parity tests.  The reference walks, for location
+// loop l and selected bucket J, the positions of the range
+// [ceil((J-1/2) n/B), ceil((J+1/2) n/B)) in the classes of the comb residues,
+// adds one to score[h] of the frequency h = position * a mod n and appends h
+// when its score reaches thr.  Those frequencies are the located set
+// {j W + c}; a frequency's score after loop l is the number of loops <= l
+// whose selected buckets hold h * ai mod n.  One thread per located
+// frequency (prefill order) writes its append key -- (loop where the score
+// reaches thr, rank of that bucket in J, offset of the position in the
+// range), the order of the reference's appends -- or -1.
+__global__ void __launch_bounds__(kBlock) k_comb_vote_keys(V12Launch a, int64_t chunks,
+                                                          int64_t *keys, int64_t stride)
+{
+  extern __shared__ unsigned selbits[];
+  const int s = (int)(blockIdx.x / chunks);
+  const int64_t t = (blockIdx.x % chunks) * kBlock + threadIdx.x;
+  const int B = a.B_loc, Lloc = a.Lloc, num = a.num;
+  const int words = (B + 31) / 32;
+  const int *J = a.J + (int64_t)s * Lloc * num;
+  for (int i = threadIdx.x; i < Lloc * words; i += blockDim.x) selbits[i] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < Lloc * num; i += blockDim.x) {
+    const int l = i / num, b = J[i];
+    atomicOr(&selbits[l * words + (b >> 5)], 1u << (b & 31));
+  }
+  __syncthreads();
+  const int nc = a.num_comb[s];
+  const int64_t located = (int64_t)nc * (a.n / a.W);
+  if (t >= a.est_cap) return;
+  if (t >= located) {  // capacity past the located set (several comb loops)
+    keys[(int64_t)s * stride + 2 * t] = -1;
+    keys[(int64_t)s * stride + 2 * t + 1] = -1;
+    return;
+  }
+  const int *prm = a.params + (int64_t)s * (2 * a.L + max(a.comb_loops, 1));
+  const unsigned mask = (unsigned)a.n - 1;
+  const int m = a.n / B, half = m / 2, logm = ilog2(m);
+  const unsigned h = (unsigned)(t / nc) * (unsigned)a.W +
+                     (unsigned)a.approved[(int64_t)s * a.comb_loops * num + t % nc];
+  int64_t key = -1;
+  int score = 0;
+  for (int l = 0; l < Lloc && key < 0; l++) {
+    const unsigned p = (h * (unsigned)prm[a.L + l]) & mask;
+    const unsigned bk = ((p + (unsigned)half) & mask) >> logm;
+    const unsigned *sb = selbits + l * words;
+    if (!((sb[bk >> 5] >> (bk & 31)) & 1)) continue;
+    if (++score < a.thr) continue;
+    int rank = __popc(sb[bk >> 5] & ((1u << (bk & 31)) - 1));
+    for (unsigned wd = 0; wd < (bk >> 5); wd++) rank += __popc(sb[wd]);
+    const unsigned low = (bk * (unsigned)m - (unsigned)half + (unsigned)a.n) & mask;
+    key = (int64_t)l << 48 | (int64_t)rank << 28 | (int64_t)((p - low) & mask);
+  }
+  keys[(int64_t)s * stride + 2 * t] = key;
+  keys[(int64_t)s * stride + 2 * t + 1] = (int64_t)h;
+}
+
 // ---------------------------------------------------------------- estimate
 // Per located frequency h and loop l: the bucket h hashes to, divided by the
 // filter response at h's offset from the bucket centre (a half-bucket tie
@@ -1330,6 +1389,24 @@ int select_vote_v1(const V12Launch &a, hipStream_t st)
   if (launched("vote_scan") != 0) return -1;
   hipLaunchKernelGGL(k_vote_write, grid, dim3(kBlock), lb, st, a, chunks);
   return launched("vote_write");
+}
+
+int comb_vote_keys(const V12Launch &a, int64_t *keys, int64_t stride, hipStream_t st)
+{
+  if (a.B_loc <= kMaxLdsSelect) {
+    allow_lds(k_select<false>, (size_t)a.B_loc * sizeof(double));
+    hipLaunchKernelGGL(k_select<false>, dim3(a.S * a.Lloc), dim3(kBlock),
+                       (size_t)a.B_loc * sizeof(double), st, a);
+  } else {
+    hipLaunchKernelGGL(k_select<true>, dim3(a.S * a.Lloc), dim3(kBlock), 0, st, a);
+  }
+  if (launched("select (v2 votes)") != 0) return -1;
+  const int64_t chunks = (a.est_cap + kBlock - 1) / kBlock;
+  const size_t lb = (size_t)a.Lloc * ((a.B_loc + 31) / 32) * 4;
+  allow_lds(k_comb_vote_keys, lb);
+  hipLaunchKernelGGL(k_comb_vote_keys, dim3((unsigned)((int64_t)a.S * chunks)), dim3(kBlock), lb,
+                     st, a, chunks, keys, stride);
+  return launched("comb_vote_keys");
 }
 
 // Blocks that keep every CU busy with several wavefronts.

@@ -289,6 +289,22 @@ class sfft:
             raise SfftError(_lib.last_error())
         return buf[:m].copy()
 
+    def debug_comb_votes(self, enabled=True):
+        """v2 FP64 plans: count the location votes in later execs."""
+        if self._L.sfft_plan_debug_comb_votes(self.sfft_plan, int(bool(enabled))) != 0:
+            raise SfftError(_lib.last_error())
+
+    def comb_votes(self, signal=0):
+        """Frequencies the reference's v2 location votes append to its hit
+        list, in its order (src/computefourier-1.0-2.0.cc:125-184), for
+        signal `signal` of the last batch."""
+        m = self._L.sfft_debug_comb_votes(self.sfft_plan, signal, None, 0)
+        if m < 0:
+            raise SfftError(_lib.last_error())
+        buf = np.zeros(max(1, m), np.int32)
+        self._L.sfft_debug_comb_votes(self.sfft_plan, signal, buf.ctypes.data_as(C.c_void_p), m)
+        return buf[:m].copy()
+
     def execute_sparse_host(self, x):
         """One host signal, sparse result on the host: (freqs, values)."""
         x = np.require(np.asanyarray(x), np.complex128, ['C', 'ALIGNED'])

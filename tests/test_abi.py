@@ -175,3 +175,38 @@ def test_tools_usage_without_gpu(capsys):
     assert "timing_many" in capsys.readouterr().out
     with pytest.raises(SystemExit):
         tools.main(["timing", "-h"])
+
+
+def _build_c_caller(out_dir):
+    """gcc build of tests/c/sfft_caller.c (the reference's test/test.c call
+    sequence) against include/sfft.h, linked with libsfft_amd.so."""
+    import subprocess
+    exe = os.path.join(str(out_dir), "sfft_caller")
+    cmd = ["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-O2",
+           "-I" + os.path.join(ROOT, "include"), "-o", exe,
+           os.path.join(ROOT, "tests", "c", "sfft_caller.c"),
+           "-L" + os.path.dirname(_lib.LIB_PATH), "-lsfft_amd",
+           "-Wl,-rpath," + os.path.dirname(_lib.LIB_PATH), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return exe
+
+
+def test_c_caller_compiles_and_links(tmp_path):
+    """The header's C branch (complex_t = double complex) compiles warning-free
+    and a C program links against the library's exports."""
+    exe = _build_c_caller(tmp_path)
+    assert os.path.getsize(exe) > 0
+
+
+def test_cpp_includes_header(tmp_path):
+    """The C++ branch (complex_t = std::complex<double>) compiles too."""
+    import subprocess
+    src = tmp_path / "t.cc"
+    src.write_text('#include "sfft.h"\n#include "sfft_server.h"\n'
+                   'int main() { complex_t z(1, 2); sfft_output_c o{0, 0, nullptr, nullptr};\n'
+                   '  return (int)(z.real() + (double)o.count) - 1; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                        "-I" + os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -157,3 +157,93 @@ def test_rejects_host_pointers(torch_cuda):
             assert what in _lib.last_error()
         st = srv.stats()
     assert st["rejected"] == 2 and st["submitted"] == 0 and st["completed"] == 0
+
+
+def _cfg5_draw(count, seed):
+    """A seeded draw of the serving workload: n uniform on {2^18, 2^20, 2^22,
+    2^24}, k = floor(10 * 200^U), v2 (SNR 30 dB) or v3 (noiseless) 50/50;
+    v2 draws the reference cannot run (W_Comb < 2k+1) are redrawn."""
+    rng = np.random.default_rng(seed)
+    sizes = [1 << 18, 1 << 20, 1 << 22, 1 << 24]
+    reqs = []
+    while len(reqs) < count:
+        n = int(sizes[rng.integers(4)])
+        k = int(np.floor(10 * 200 ** rng.random()))
+        v = 2 if rng.random() < 0.5 else 3
+        try:
+            output_capacity(n, k, v)
+        except sfft_amd.SfftError:
+            continue
+        reqs.append((v, n, k))
+    return reqs
+
+
+def test_cfg5_mix_equals_alone(torch_cuda):
+    """The serving workload, Poisson arrivals at 2000/s into 3 workers:
+    v3 up to k = 2000 (W_Man = 16384, above the LDS transform size), v2 at
+    the feasibility edge (k = 2^j - 1 with W_Comb = 2k + 2), both
+    precisions.  Every result equals the same request run alone (exact
+    equality of located set and values), batches overlap on the device, and
+    four small requests are checked against the oracle."""
+    import oracle
+    torch = torch_cuda
+    reqs = [(v, n, k, "fp32_refine" if i % 2 else "fp64")
+            for i, (v, n, k) in enumerate(_cfg5_draw(26, 555))]
+    # the edges: v3 with k > 1000, v2 with B = W_Comb = 2k + 2 (k = 511 at 2^22)
+    reqs += [(3, 1 << 24, 1800, "fp64"), (3, 1 << 22, 1200, "fp64"),
+             (2, 1 << 22, 511, "fp32_refine"), (2, 1 << 18, 255, "fp64"),
+             (2, 1 << 18, 30, "fp64"), (3, 1 << 18, 40, "fp64")]
+    xs, seeds = [], []
+    for i, (v, n, k, prec) in enumerate(reqs):
+        x, _, _ = signals_device(n, k, [7000 + i], snr_db=30.0 if v == 2 else None)
+        xs.append(x[0].contiguous())
+        seeds.append((2000 + 11 * i, 300 + 13 * i))
+    arrivals = np.cumsum(np.random.default_rng(9).exponential(1 / 2000.0, len(reqs)))
+    import time
+    with Server(workers=3, max_batch=8) as srv:
+        for key in sorted({(n, k, v, p) for (v, n, k, p) in reqs}):
+            srv.prepare(*key)
+        t0 = time.perf_counter()
+        tickets = []
+        for i, (v, n, k, prec) in enumerate(reqs):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(xs[i], k, v, prec, seeds[i]))
+        got = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["max_concurrent_batches"] >= 2
+    for i, (v, n, k, prec) in enumerate(reqs):
+        print(f"case cfg5 v{v} n={n} k={k} {prec}", flush=True)
+        f1, v1 = _alone(torch, v, n, k, prec, xs[i], seeds[i])
+        assert np.array_equal(got[i][0].cpu().numpy(), f1)
+        assert np.array_equal(got[i][1].cpu().numpy(), v1)
+    # oracle checks of the last four (small n): same filters, same draws
+    for i in range(len(reqs) - 4, len(reqs)):
+        v, n, k, prec = reqs[i]
+        if prec != "fp64":
+            continue
+        plan = sfft_amd.sfft(n, k, v, strict=False, seed=seeds[i])
+        t1, r1 = plan.filter(0)
+        t2, r2 = plan.filter(1)
+        x = xs[i].cpu().numpy()
+        oracle.seed_glibc(*seeds[i])
+        fr = got[i][0].cpu().numpy()
+        va = got[i][1].cpu().numpy()
+        if v == 3:
+            P = oracle.V3(n, k)
+            P.set_filters(t1, r1, t2, r2)
+            ref = P.exec(x, P.draw())
+            if ref["early"]:
+                assert len(fr) == 0
+                continue
+            assert np.array_equal(fr, ref["freqs"])
+            assert np.abs(va - ref["vals"]).max() <= 1e-9 * max(1.0, np.abs(ref["vals"]).max())
+        else:
+            P = oracle.V12(n, k, True)
+            P.set_filters(t1, r1, t2, r2)
+            a, ai, off = P.draw()
+            tr = P.exec(x, a, ai, off, trace=True)
+            assert np.array_equal(np.unique(fr), np.unique(tr["hits"][:tr["hits_prefill"]]))
+            assert np.array_equal(va, tr["out"][fr])

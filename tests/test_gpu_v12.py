@@ -360,3 +360,27 @@ def test_concurrent_calls_on_one_plan(torch_cuda):
         return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     match = [[same(g, r) for r in ref] for g in got]
     assert (match[0][0] and match[1][1]) or (match[0][1] and match[1][0]), match
+
+
+@pytest.mark.parametrize("n,k,snr", [(1 << 14, 50, None), (1 << 16, 50, 20.0),
+                                     (1 << 20, 50, 30.0), (1 << 22, 64, 40.0)])
+def test_v2_comb_votes_match_oracle(torch_cuda, n, k, snr):
+    """The v2 location votes (skipped by the exec: they never change the
+    output), counted on the GPU on demand, append the oracle's hits in the
+    oracle's order (hits after the comb prefill; src/computefourier-1.0-2.0.cc
+    125-184), and the output is unchanged by counting them."""
+    print(f"case comb votes n={n} k={k} snr={snr}", flush=True)
+    x, f, v = signal_np(n, k, seed=8800 + k, snr_db=snr)
+    plan = _gpu_plan(2, n, k)
+    plain = _gpu_plan(2, n, k)
+    plan.debug_comb_votes(True)
+    fr, va = plan.execute_sparse_host(x)
+    f0, v0 = plain.execute_sparse_host(x)
+    assert np.array_equal(fr, f0) and np.array_equal(va, v0)
+    got = plan.comb_votes(0)
+    ref = _oracle_run(2, n, k, x, _plan_filters(plan))
+    appended = ref["hits"][ref["hits_prefill"]:ref["hits_found"]]
+    assert ref["hits_found"] - ref["hits_prefill"] == len(got)
+    assert np.array_equal(got, appended)
+    # every appended frequency duplicates a prefill entry
+    assert np.all(np.isin(appended, ref["hits"][:ref["hits_prefill"]]))
