@@ -94,8 +94,13 @@ typedef struct sfft_output_c {
 typedef enum sfft_precision {
   /* FP64 throughout; reproduces the reference's arithmetic order. */
   SFFT_PRECISION_FP64 = 0,
-  /* FP32 bucketization and bucket FFTs (compensated FP32 sums), FP64 comb
-   * selection and FP64 estimation of the located values. */
+  /* v1/v2: FP32 bucketization and bucket FFTs (compensated FP32 sums), FP64
+   * comb selection and FP64 estimation of the located values.  v3: the
+   * reference's FP64 location (FP32 phases cannot locate at n = 2^24), then
+   * the located values re-estimated in FP64 by least squares over fresh
+   * Mansour buckets at 6 strided offsets with integer-exact phases, with up
+   * to two unlocated tones per bucket found by a Prony step (located set
+   * unchanged; false positives come out near zero). */
   SFFT_PRECISION_FP32_REFINE = 1
 } sfft_precision;
 
@@ -217,7 +222,8 @@ int sfft_derive_params(int n, int k, sfft_version version, int *p10);
  * the reference's early return (src/computefourier-3.0.cc:859-860, empty
  * result), bit 1 result map overflow (the call failed), bit 2 the
  * estimate/subtract loop stopped at its 3*4096-pass cap instead of at
- * repeated bucket counts.  Copies up to num flags; returns the number of
+ * repeated bucket counts, bit 3 the refine form kept the reference's values
+ * in a bucket it could not solve.  Copies up to num flags; returns the number of
  * signals recorded (0 for v1/v2 plans), or -1 for a bad plan. */
 int sfft_plan_last_status(const sfft_plan *plan, int *flags, int num);
 

@@ -242,9 +242,17 @@ int64_t Plan::plan_bytes() const
 
 int64_t Plan::bytes_per_signal() const
 {
-  if (version == SFFT_VERSION_3)
-    return (v3.W_Man > kMaxLdsFft64 ? 96LL : 32LL) * v3.W_Man + (int64_t)v3_Mcap * 28 + (int64_t)v3_Rcap * 236 +
+  if (version == SFFT_VERSION_3) {
+    // Mansour buckets 2W; above the LDS size the pre-pass copy and its
+    // transform scratch (2 or, refined, kV3RefineOffsets + 1 offsets each);
+    // refined below it, the fresh buckets at kV3RefineOffsets offsets
+    const bool ref = precision == SFFT_PRECISION_FP32_REFINE;
+    const bool big = v3.W_Man > kMaxLdsFft64;
+    const int64_t off = ref ? kV3RefineOffsets : 0;
+    const int64_t ms = 32LL + (big ? 32LL * (ref ? off + 1 : 2) : 16LL * off);
+    return ms * v3.W_Man + (int64_t)v3_Mcap * 28 + (int64_t)v3_Rcap * 236 +
            (int64_t)v3_Fcap * 40 + 16LL * (v3.w_g2 + 2) + 512;
+  }
   const V12Params &p = v12;
   const bool fp32 = precision == SFFT_PRECISION_FP32_REFINE;
   const bool need64 = !fp32 || !p.comb;
@@ -583,9 +591,14 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   const int Pcap = q.w_g2 + 2;
   const size_t b_ph = al(Sz * Pcap * 16);
   const bool big_ms = q.W_Man > kMaxLdsFft64;
-  const size_t b_pre = big_ms ? al(Sz * 2 * q.W_Man * 16) : 0;
+  // the refine form: fresh Mansour buckets at kV3RefineOffsets offsets (in
+  // the pre-pass arrays when W is above the LDS size)
+  const bool refine = P->precision == SFFT_PRECISION_FP32_REFINE;
+  const int ms_loops = refine && big_ms ? kV3RefineOffsets + 1 : 2;
+  const size_t b_pre = big_ms ? al(Sz * ms_loops * q.W_Man * 16) : 0;
+  const size_t b_ref = refine && !big_ms ? al(Sz * kV3RefineOffsets * q.W_Man * 16) : 0;
   const size_t total = b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi +
-                       b_ph + 2 * b_pre;
+                       b_ph + 2 * b_pre + b_ref;
   if (!P->ws->params.ensure(param_bytes) || !P->ws->ptrs.ensure(4 * tab_bytes) ||
       !P->ws->v3.ensure(total) || !P->ws->count.ensure(cnt_bytes) ||
       !P->ws->v3_status.ensure((size_t)S * 4))
@@ -643,6 +656,20 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   a.remi = reinterpret_cast<int *>(w);
   w += b_remi;
   a.phase = reinterpret_cast<double2 *>(w);
+  w += b_ph;
+  a.ms_loops = ms_loops;
+  a.refine = refine ? 1 : 0;
+  a.ref_ms = refine && !big_ms ? reinterpret_cast<double2 *>(w) : nullptr;
+  {
+    // refine offset stride: odd, near 0.618 n/W, below n/W
+    const int np = n / q.W_Man;
+    int t = ((int)std::lround(np * 0.6180339887498949)) | 1;
+    if (t >= np) t = np - 1 > 0 ? np - 1 : 1;
+    unsigned inv = (unsigned)t;  // Newton: t inv = 1 mod 2^32 after 5 steps
+    for (int it = 0; it < 5; it++) inv *= 2u - (unsigned)t * inv;
+    a.ref_t = t;
+    a.ref_tinv = (int)(inv & (unsigned)(np - 1));
+  }
   a.Pcap = Pcap;
   a.Mcap = P->v3_Mcap;
   a.Hcap = P->v3_Hcap;

@@ -22,6 +22,8 @@ using cd = std::complex<double>;
 constexpr int kMaxLdsFft64 = 8192;
 // Largest number of loops (location + estimation) the median network takes.
 constexpr int kMaxLoops = 32;
+// Mansour offsets of the v3 refine form (equations per bucket).
+constexpr int kV3RefineOffsets = 6;
 
 void set_error(const std::string &msg);
 
@@ -358,8 +360,20 @@ struct V3Launch {
   int tw_nmax;
   // per-signal workspace, strided by signal
   double2 *ms;               // 2W (interleaved loops)
-  double2 *ms_pre, *ms_tmp;  // W above the LDS size: S x 2 x W transforms
-                             // computed before k_v3_multi (else null)
+  double2 *ms_pre, *ms_tmp;  // W above the LDS size: S x ms_loops x W
+                             // transforms computed before k_v3_multi (else null)
+  int ms_loops;              // Mansour offsets in ms_pre: the reference's
+                             // two (init_offset + 0, 1), then with refine
+                             // init_offset + l ref_t, 1 <= l < kV3RefineOffsets
+  // refine form: the located values re-estimated from fresh Mansour buckets
+  // at offsets init_offset + l ref_t, l < kV3RefineOffsets (S x offsets x W
+  // in ref_ms; rows 0, 2, 3, ... of ms_pre when W is above the LDS size).
+  // ref_t is odd, near 0.618 n/W: members of a bucket m W apart sit at
+  // phase steps m ref_t / (n/W) mod 1, well apart for the small m at which
+  // the reference places its false positives; ref_tinv = ref_t^-1 mod n/W.
+  int refine;
+  int ref_t, ref_tinv;
+  double2 *ref_ms;
   int *map_keys;             // Mcap (insertion order)
   double2 *map_vals;         // Mcap
   int *map_hidx;             // Hcap hash slots -> entry
@@ -374,7 +388,8 @@ struct V3Launch {
   int Pcap;
   int max_passes;
   int64_t *count;            // S located counts
-  int *status;               // S: 1 early return, 2 map full, 4 pass cap
+  int *status;               // S: 1 early return, 2 map full, 4 pass cap,
+                             // 8 refine kept some reference values
   int32_t *const *o_freq;
   double2 *const *o_val;
   double2 *const *o_dense;

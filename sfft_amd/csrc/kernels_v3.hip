@@ -551,6 +551,262 @@ __device__ __forceinline__ bool map_add_found(const Ctx &c, int F, int *sh, int 
 }  // namespace
 
 // The whole of one signal's transform, in one workgroup.
+// ---------------------------------------------------------------- refine
+// The refine form re-estimates the values of the located frequencies after
+// the reference's loop, keeping the located set.  The Mansour filter is an
+// exact aliasing: at offset g its bucket j is
+//   M_g[j] = W * sum over f = j (mod W) of X[f] e^{2 pi i f g / n},
+// so fresh buckets at the offsets g0 + l t (l < kRefOff, t = ref_t) give,
+// per bucket, kRefOff equations in the values of the frequencies of that
+// residue class.  Members f = j + W q enter through the nodes
+// e^{2 pi i q t / (n/W)}; with t odd near 0.618 n/W, members a few W apart
+// (where the reference's false positives sit) get well separated nodes,
+// which consecutive offsets (t = 1) would crowd within 2 pi m / (n/W).
+// With the located members of bucket j (at most kRefMax) as unknowns the
+// least-squares solution (QR) is exact up to rounding; every phase is reduced in
+// integers, f (g0 + l t) mod n, before its sincos, instead of the reference's
+// FP64 product 2 pi a g f / n (cf3 497, 591-593), whose rounding reaches a
+// quarter radian at n = 2^24.  A bucket whose residual stays large holds
+// unlocated tones: after annihilating the located members the sequence d_l
+// is a sum of r exponentials w_u^l, w_u = e^{2 pi i f_u t / n}; for r = 1
+// w_u = d_{l+1} / d_l, for r = 2 the roots of the linear predictor
+// d_{l+2} + c1 d_{l+1} + c0 d_l = 0 (a Prony step).  Each w_u is snapped to
+// the residue class and added as an unknown; if the residual is still large
+// the bucket keeps the reference's values (status bit 8).
+constexpr int kRefOff = kV3RefineOffsets;  // Mansour offsets: equations per bucket
+constexpr int kRefMax = 5;  // unknowns per bucket (located + unlocated)
+
+__device__ __forceinline__ double2 cis_mod(long long r, int n)
+{
+  double sn, cs;
+  sincos(2 * M_PI * (double)r / (double)n, &sn, &cs);
+  return make_double2(cs, sn);
+}
+
+__device__ __forceinline__ double2 cconjmul(double2 a, double2 b)  // conj(a) * b
+{
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ double2 csqrt_d(double2 z)
+{
+  const double r = hypot(z.x, z.y);
+  const double re = sqrt(0.5 * (r + z.x)), im = sqrt(0.5 * (r - z.x));
+  return make_double2(re, z.y < 0 ? -im : im);
+}
+
+// The frequency of residue class j whose node e^{2 pi i f t / n} is nearest
+// the angle ang: f t = j t + W (q t mod n/W) (mod n), so the nearest
+// multiple of W of (n ang / 2 pi - j t) mod n gives q t mod n/W, and tinv
+// undoes the stride.
+__device__ __forceinline__ long long ref_snap(double ang, int j, int n, int W, int t, int tinv)
+{
+  const long long fe = llrint(ang * n / (2 * M_PI));
+  long long r = (fe - (long long)j * t) % n;
+  if (r < 0) r += n;
+  const long long np1 = n / W - 1;
+  const long long qt = llrint((double)r / W) & np1;
+  return j + (long long)W * ((qt * tinv) & np1);
+}
+
+// Least squares over kRefOff equations: y[l] ~ sum_m X[m] e[m][l], p <=
+// kRefMax unknowns, by modified Gram-Schmidt QR of the columns e[m] (the
+// columns are samples of unit exponentials; QR keeps their conditioning
+// instead of squaring it).  Returns the squared residual (1e300 when the
+// columns are dependent).
+__device__ double ref_solve(const double2 (&e)[kRefMax][kRefOff], const double2 (&y)[kRefOff],
+                           int p, double2 (&X)[kRefMax])
+{
+  double2 Q[kRefMax][kRefOff], R[kRefMax][kRefMax], z[kRefMax];
+  for (int m = 0; m < p; m++) {
+    for (int l = 0; l < kRefOff; l++) Q[m][l] = e[m][l];
+    for (int q = 0; q < m; q++) {
+      double2 d = make_double2(0, 0);
+      for (int l = 0; l < kRefOff; l++) d = cadd(d, cconjmul(Q[q][l], Q[m][l]));
+      R[q][m] = d;
+      for (int l = 0; l < kRefOff; l++) Q[m][l] = csub(Q[m][l], cmul(d, Q[q][l]));
+    }
+    double nn = 0;
+    for (int l = 0; l < kRefOff; l++) nn += Q[m][l].x * Q[m][l].x + Q[m][l].y * Q[m][l].y;
+    if (!(nn > 1e-24)) return 1e300;
+    const double inv = 1.0 / sqrt(nn);
+    R[m][m] = make_double2(sqrt(nn), 0);
+    for (int l = 0; l < kRefOff; l++) Q[m][l] = make_double2(Q[m][l].x * inv, Q[m][l].y * inv);
+  }
+  for (int m = 0; m < p; m++) {
+    double2 d = make_double2(0, 0);
+    for (int l = 0; l < kRefOff; l++) d = cadd(d, cconjmul(Q[m][l], y[l]));
+    z[m] = d;
+  }
+  for (int m = p - 1; m >= 0; m--) {
+    double2 acc = z[m];
+    for (int q = m + 1; q < p; q++) acc = csub(acc, cmul(R[m][q], X[q]));
+    X[m] = make_double2(acc.x / R[m][m].x, acc.y / R[m][m].x);
+  }
+  double res = 0;
+  for (int l = 0; l < kRefOff; l++) {
+    double2 r = y[l];
+    for (int m = 0; m < p; m++) r = csub(r, cmul(X[m], e[m][l]));
+    res += r.x * r.x + r.y * r.y;
+  }
+  return res;
+}
+
+// The refine pass of one signal after its loop (all threads; contains
+// barriers).  M located entries in c.keys / c.vals (insertion order).
+__device__ void refine_values(const Ctx &c, const double2 *x, int M, double2 *lds, int *s_flag)
+{
+  const V3Launch &a = *c.a;
+  const int n = c.n, W = c.W, t = a.ref_t, tinv = a.ref_tinv;
+  const unsigned mask = (unsigned)n - 1;
+  // row l: the buckets at offset g0 + l t (rows 0, 2, 3, ... of the
+  // pre-pass, whose row 1 is the reference's g0 + 1)
+  auto row = [&](int l) -> const double2 * {
+    return a.ms_pre ? a.ms_pre + ((int64_t)c.s * a.ms_loops + (l ? l + 1 : 0)) * W
+                    : a.ref_ms + ((int64_t)c.s * kRefOff + l) * W;
+  };
+  if (!a.ms_pre) {
+    const int sigma = n / W, logW = ilog2v(W);
+    double2 *RM = a.ref_ms + (int64_t)c.s * kRefOff * W;  // [l][j]
+    for (int l = 0; l < kRefOff; l++) {
+      const unsigned off = (unsigned)c.g0 + (unsigned)l * (unsigned)t;
+      for (int i = threadIdx.x; i < W; i += blockDim.x)
+        lds[bitrev((unsigned)i, logW)] = x[(off + (unsigned)i * (unsigned)sigma) & mask];
+      __syncthreads();
+      lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
+      for (int i = threadIdx.x; i < W; i += blockDim.x) RM[l * W + i] = lds[i];
+      __syncthreads();
+    }
+  }
+  // the located entries of every residue class: linked lists through
+  // head (W entries, in cf) and next (M entries, in remi)
+  int *head = c.cf, *next = c.remi;
+  for (int j = threadIdx.x; j < W; j += blockDim.x) head[j] = -1;
+  __threadfence_block();
+  __syncthreads();
+  for (int e = threadIdx.x; e < M; e += blockDim.x)
+    next[e] = atomicExch(&head[c.keys[e] & (W - 1)], e);
+  __threadfence_block();
+  __syncthreads();
+  int kept = 0;
+  for (int j = threadIdx.x; j < W; j += blockDim.x) {
+    int mem[kRefMax];
+    int p = 0;
+    for (int e = head[j]; e >= 0; e = next[e]) {
+      if (p == kRefMax) {
+        p = kRefMax + 1;
+        break;
+      }
+      mem[p++] = e;
+    }
+    if (p == 0) continue;
+    if (p > kRefMax || p >= kRefOff) {  // no spare equation: keep the reference values
+      kept = 1;
+      continue;
+    }
+    // members by frequency (the list order depends on the atomics)
+    for (int u = 1; u < p; u++)
+      for (int v = u; v > 0 && c.keys[mem[v]] < c.keys[mem[v - 1]]; v--) {
+        const int tmp = mem[v];
+        mem[v] = mem[v - 1];
+        mem[v - 1] = tmp;
+      }
+    double2 y[kRefOff], e[kRefMax][kRefOff], X[kRefMax], w[kRefMax];
+    double scale = 0;
+    for (int l = 0; l < kRefOff; l++) {
+      const double2 m = row(l)[j];
+      y[l] = make_double2(m.x / W, m.y / W);
+      scale += y[l].x * y[l].x + y[l].y * y[l].y;
+    }
+    auto column = [&](int m, long long f) {
+      for (int l = 0; l < kRefOff; l++)
+        e[m][l] = cis_mod((f * ((long long)c.g0 + (long long)l * t)) & mask, n);
+    };
+    for (int m = 0; m < p; m++) {
+      const long long f = c.keys[mem[m]];
+      column(m, f);
+      w[m] = cis_mod((f * t) & mask, n);
+    }
+    // residual at most 1e-10 of the bucket (or of 1e-4, below which a
+    // bucket holds rounding only)
+    const double tol = 1e-20 * (scale > 1e-8 ? scale : 1e-8);
+    double res = ref_solve(e, y, p, X);
+    if (res > tol) {
+      // annihilate the members' nodes: d_l <- d_{l+1} - w_m d_l, once per member
+      double2 d[kRefOff];
+      int len = kRefOff;
+      for (int l = 0; l < kRefOff; l++) d[l] = y[l];
+      for (int m = 0; m < p; m++) {
+        for (int l = 0; l + 1 < len; l++) d[l] = csub(d[l + 1], cmul(w[m], d[l]));
+        len--;
+      }
+      double2 X2[kRefMax];
+      if (p + 1 <= kRefMax && len >= 2) {  // one unlocated tone: w_u = d_{l+1} / d_l
+        double2 num = make_double2(0, 0);
+        double den = 0;
+        for (int l = 0; l + 1 < len; l++) {
+          num = cadd(num, cconjmul(d[l], d[l + 1]));
+          den += d[l].x * d[l].x + d[l].y * d[l].y;
+        }
+        if (den > 0) {
+          column(p, ref_snap(atan2(num.y, num.x), j, n, W, t, tinv));
+          const double res2 = ref_solve(e, y, p + 1, X2);
+          if (res2 <= tol) {
+            res = res2;
+            for (int m = 0; m < p; m++) X[m] = X2[m];
+          }
+        }
+      }
+      if (res > tol && p + 2 <= kRefMax && len >= 4) {
+        // two unlocated tones: least-squares predictor over l < len - 2,
+        // normal equations of the columns (d_{l+1}, d_l) against -d_{l+2}
+        double g11 = 0, g22 = 0;
+        double2 g12 = make_double2(0, 0), b1 = make_double2(0, 0), b2 = make_double2(0, 0);
+        for (int l = 0; l + 2 < len; l++) {
+          g11 += d[l + 1].x * d[l + 1].x + d[l + 1].y * d[l + 1].y;
+          g22 += d[l].x * d[l].x + d[l].y * d[l].y;
+          g12 = cadd(g12, cconjmul(d[l + 1], d[l]));
+          b1 = csub(b1, cconjmul(d[l + 1], d[l + 2]));
+          b2 = csub(b2, cconjmul(d[l], d[l + 2]));
+        }
+        const double det = g11 * g22 - (g12.x * g12.x + g12.y * g12.y);
+        if (det > 0) {
+          const double2 g21 = make_double2(g12.x, -g12.y);
+          const double2 n1 = csub(make_double2(g22 * b1.x, g22 * b1.y), cmul(g12, b2));
+          const double2 n0 = csub(make_double2(g11 * b2.x, g11 * b2.y), cmul(g21, b1));
+          const double2 c1 = make_double2(n1.x / det, n1.y / det);
+          const double2 c0 = make_double2(n0.x / det, n0.y / det);
+          const double2 sq = csqrt_d(csub(cmul(c1, c1), make_double2(4 * c0.x, 4 * c0.y)));
+          const double2 z1 = make_double2(0.5 * (sq.x - c1.x), 0.5 * (sq.y - c1.y));
+          const double2 z2 = make_double2(-0.5 * (sq.x + c1.x), -0.5 * (sq.y + c1.y));
+          const long long u1 = ref_snap(atan2(z1.y, z1.x), j, n, W, t, tinv);
+          const long long u2 = ref_snap(atan2(z2.y, z2.x), j, n, W, t, tinv);
+          if (u1 != u2) {
+            column(p, u1);
+            column(p + 1, u2);
+            const double res2 = ref_solve(e, y, p + 2, X2);
+            if (res2 <= tol) {
+              res = res2;
+              for (int m = 0; m < p; m++) X[m] = X2[m];
+            }
+          }
+        }
+      }
+    }
+    if (res <= tol) {
+      for (int m = 0; m < p; m++) c.vals[mem[m]] = X[m];
+    } else {
+      kept = 1;
+    }
+  }
+  if (kept) atomicOr(s_flag, 8);
+  __threadfence_block();
+  __syncthreads();
+}
+
+__device__ void write_result(const V3Launch &a, int s, const int *keys, const double2 *vals,
+                             int M, int flags, double2 *lds);
+
 __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
 {
   extern __shared__ double2 lds[];
@@ -597,7 +853,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
 
   // ---- Mansour filter (cf3 96-127): two aliased sub-samplings, W-pt FFTs
   if (a.ms_pre) {
-    const double2 *pre = a.ms_pre + (int64_t)s * 2 * W;
+    const double2 *pre = a.ms_pre + (int64_t)s * a.ms_loops * W;
     for (int i = threadIdx.x; i < 2 * W; i += blockDim.x) c.MS[2 * (i % W) + i / W] = pre[i];
   } else {
     const int sigma = n / W, logW = ilog2v(W);
@@ -764,16 +1020,30 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   }
   if (pass >= a.max_passes && threadIdx.x == 0) s_flag |= 4;
   __syncthreads();
+  if (a.refine) {  // k_v3_refine refines the map and writes the result
+    if (threadIdx.x == 0) {
+      a.count[s] = s_size;
+      a.status[s] = s_flag;
+    }
+    return;
+  }
+  write_result(a, s, c.keys, c.vals, s_size, s_flag, lds);
+}
 
-  // ---- result: the map sorted by frequency
-  const int M = s_size;
+// The result of signal s: the map (M entries in insertion order) sorted by
+// frequency into the sparse outputs and scattered into the dense one; count
+// and status flags.  All threads; contains barriers.
+__device__ void write_result(const V3Launch &a, int s, const int *keys, const double2 *vals,
+                             int M, int flags, double2 *lds)
+{
+
   int P = 1;
   while (P < M) P <<= 1;
   uint64_t *srt = reinterpret_cast<uint64_t *>(lds);
   const bool sortable = (size_t)P * 8 <= a.lds_bytes;
   if (sortable) {
     for (int i = threadIdx.x; i < P; i += blockDim.x)
-      srt[i] = i < M ? (((uint64_t)(unsigned)c.keys[i] << 32) | (unsigned)i) : ~0ull;
+      srt[i] = i < M ? (((uint64_t)(unsigned)keys[i] << 32) | (unsigned)i) : ~0ull;
     __syncthreads();
     for (int k2 = 2; k2 <= P; k2 <<= 1) {
       for (int j = k2 >> 1; j > 0; j >>= 1) {
@@ -794,8 +1064,8 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   }
   for (int t = threadIdx.x; t < M; t += blockDim.x) {
     const int e = sortable ? (int)(srt[t] & 0xffffffffu) : t;
-    const int key = c.keys[e];
-    const double2 v = c.vals[e];
+    const int key = keys[e];
+    const double2 v = vals[e];
     if (a.o_freq && t < a.o_cap) {
       int32_t *of = a.o_freq[s];
       double2 *ov = a.o_val[s];
@@ -806,7 +1076,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   }
   if (threadIdx.x == 0) {
     a.count[s] = M;
-    a.status[s] = s_flag;
+    a.status[s] = flags;
   }
 }
 
@@ -821,30 +1091,65 @@ __global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *__restric
   v3_signal(groups[b.x], b.y);
 }
 
-// Sub-samples of both Mansour loops for the pre-pass: ms_pre[(s*2+l)*W + i]
-// = x[(init_offset + l + i n/W) mod n] (cf3 96-127).
+// The refine form after k_v3_multi: the same (group, signal) blocks refine
+// their map's values (refine_values) and write the result.
+__global__ void __launch_bounds__(kV3Block) k_v3_refine(const V3Launch *__restrict__ groups,
+                                                        const int2 *__restrict__ blocks)
+{
+  extern __shared__ double2 lds[];
+  __shared__ int s_flag;
+  const int2 bb = blocks[blockIdx.x];
+  const V3Launch &a = groups[bb.x];
+  if (!a.refine) return;  // written by k_v3_multi
+  const int s = bb.y;
+  const int M = (int)a.count[s];
+  if (threadIdx.x == 0) s_flag = a.status[s];
+  __syncthreads();
+  if (s_flag & 1) return;  // the reference's early return: no result
+  Ctx c;
+  c.a = &a;
+  c.s = s;
+  c.n = a.n;
+  c.W = a.W;
+  c.B1 = a.B1;
+  c.B2 = a.B2;
+  c.g0 = a.params[(int64_t)s * 6 + 4];
+  c.keys = a.map_keys + (int64_t)s * a.Mcap;
+  c.vals = a.map_vals + (int64_t)s * a.Mcap;
+  c.cf = a.cf + (int64_t)s * a.Fcap;
+  c.remi = a.remi + (int64_t)s * a.Rcap * 3;
+  refine_values(c, a.x[s], M, lds, &s_flag);
+  write_result(a, s, c.keys, c.vals, M, s_flag, lds);
+}
+
+// Sub-samples of the Mansour loops for the pre-pass: ms_pre[(s*L+l)*W + i]
+// = x[(init_offset + o_l + i n/W) mod n] (cf3 96-127), L = ms_loops, o_0 = 0
+// and o_1 = 1 the reference's two, o_l = (l - 1) ref_t after them (the
+// refine form's strided offsets).
 __global__ void k_v3_mansour_gather(V3Launch a)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)a.S * 2 * a.W) return;
+  const int L = a.ms_loops;
+  if (t >= (int64_t)a.S * L * a.W) return;
   const int i = (int)(t % a.W);
-  const int l = (int)((t / a.W) & 1);
-  const int s = (int)(t / (2 * (int64_t)a.W));
+  const int l = (int)((t / a.W) % L);
+  const int s = (int)(t / (L * (int64_t)a.W));
   const unsigned g0 = (unsigned)a.params[(int64_t)s * 6 + 4];
   const unsigned sigma = (unsigned)(a.n / a.W);
-  a.ms_pre[t] = a.x[s][(g0 + (unsigned)l + (unsigned)i * sigma) & ((unsigned)a.n - 1)];
+  const unsigned off = g0 + (l < 2 ? (unsigned)l : (unsigned)(l - 1) * (unsigned)a.ref_t);
+  a.ms_pre[t] = a.x[s][(off + (unsigned)i * sigma) & ((unsigned)a.n - 1)];
 }
 
 // Mansour pre-pass of one plan group whose W exceeds the LDS size.
 static int mansour_prepass(const V3Launch &a, hipStream_t st)
 {
   if (!a.ms_pre) return 0;
-  const int64_t total = (int64_t)a.S * 2 * a.W;
+  const int64_t total = (int64_t)a.S * a.ms_loops * a.W;
   hipLaunchKernelGGL(k_v3_mansour_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      st, a);
   if (hipGetLastError() != hipSuccess ||
-      radix2_fft64(a.ms_pre, a.ms_tmp, a.W, (int64_t)a.S * 2, 1, a.W, a.W, a.tw64, a.tw_nmax,
-                   st) != 0) {
+      radix2_fft64(a.ms_pre, a.ms_tmp, a.W, (int64_t)a.S * a.ms_loops, 1, a.W, a.W, a.tw64,
+                   a.tw_nmax, st) != 0) {
     set_error("v3: Mansour pre-pass launch failed");
     return -1;
   }
@@ -877,6 +1182,19 @@ int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks
   if (e != hipSuccess) {
     set_error(std::string("v3 multi: ") + hipGetErrorString(e));
     return -1;
+  }
+  bool refine = false;
+  for (int g = 0; g < G; g++) refine |= h_groups[g].refine != 0;
+  if (refine) {
+    // groups without refine write their result in k_v3_multi; their blocks
+    // return at once here (status and count say so)
+    allow_v3_lds(reinterpret_cast<const void *>(k_v3_refine), lds);
+    hipLaunchKernelGGL(k_v3_refine, dim3(nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error(std::string("v3 refine: ") + hipGetErrorString(e));
+      return -1;
+    }
   }
   return 0;
 }

@@ -259,7 +259,8 @@ class sfft:
 
     def last_status(self):
         """v3 flags per signal of the last exec call: 1 early return,
-        2 map overflow, 4 loop stopped at the pass cap."""
+        2 map overflow, 4 loop stopped at the pass cap, 8 the refine form
+        kept the reference's values in some bucket."""
         m = self._L.sfft_plan_last_status(self.sfft_plan, None, 0)
         if m < 0:
             raise SfftError(_lib.last_error())
