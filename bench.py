@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--only-configs", action="store_true",
                     help="run only the v1 / v2 single-signal and batched v3 configurations "
                          "(for profilers)")
+    ap.add_argument("--only-uneven", action="store_true",
+                    help="run only the v1/v2 uneven-(n, k) mix in one call (for profilers)")
     ap.add_argument("--only-fp32", action="store_true",
                     help="time only the headline form (for profilers)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -140,6 +142,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.only_configs:
         print(json.dumps({"configs": other_configs(torch, dev)}), flush=True)
+        return
+    if args.only_uneven:
+        print(json.dumps({"v12_uneven_mix": uneven_v12_mix(torch, dev)}), flush=True)
         return
     if args.calib_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
@@ -370,6 +375,71 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def uneven_v12_mix(torch, dev, reps=10):
+    """v1/v2 plans of uneven (n, k) in one sfft_exec_many_plans call: 12
+    keys (n = 2^18 ... 2^22, k = 10 ... 300), 4 signals each (SNR 30 dB,
+    FP32+refine); the call runs each plan's group of kernels in turn on one
+    stream.  Timed beside the sum of each group run alone
+    (execute_many_sparse of its 4 signals), and checked equal to those
+    alone-runs signal for signal."""
+    import sfft_amd
+    from sfft_amd.sfft import execute_many_plans
+    from sfft_amd.signals import signals_device
+    keys = [(2, 1 << 18, 10), (2, 1 << 18, 50), (2, 1 << 20, 20), (2, 1 << 20, 100),
+            (2, 1 << 20, 300), (2, 1 << 22, 10), (2, 1 << 22, 64), (2, 1 << 22, 200),
+            (1, 1 << 18, 20), (1, 1 << 20, 10), (1, 1 << 20, 50), (1, 1 << 22, 30)]
+    per = 4
+    plans = [sfft_amd.sfft(n, k, v, strict=False, precision="fp32_refine", seed=(31 + i, 4242))
+             for i, (v, n, k) in enumerate(keys)]
+    xs, order, groups = [], [], []
+    for i, (v, n, k) in enumerate(keys):
+        x, _, _ = signals_device(n, k, [5000 + 10 * i + j for j in range(per)], snr_db=30.0,
+                                 device=dev)
+        groups.append(x)
+        for j in range(per):
+            xs.append(x[j])
+            order.append(plans[i])
+    outs = [(torch.empty(p.capacity, dtype=torch.int32, device=dev),
+             torch.empty(p.capacity, dtype=torch.complex128, device=dev)) for p in order]
+    execute_many_plans(order, xs, outs)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        execute_many_plans(order, xs, outs, trim=False)
+    torch.cuda.synchronize()
+    ms_mix = (time.perf_counter() - t0) * 1e3 / reps
+    # the same groups alone, and the results of one seeded call of each
+    gouts = [p.make_sparse_outputs(per) for p in plans]
+    for p, x, (fr, va) in zip(plans, groups, gouts):
+        p.execute_many_sparse(x, fr, va)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p, x, (fr, va) in zip(plans, groups, gouts):
+            p.execute_many_sparse(x, fr, va)
+    torch.cuda.synchronize()
+    ms_alone = (time.perf_counter() - t0) * 1e3 / reps
+    for i, p in enumerate(plans):
+        p.seed(31 + i, 4242)
+    counts = execute_many_plans(order, xs, outs, trim=False)
+    for i, p in enumerate(plans):
+        p.seed(31 + i, 4242)
+    same = True
+    for i, (p, x, (fr, va)) in enumerate(zip(plans, groups, gouts)):
+        c = p.execute_many_sparse(x, fr, va)
+        for j in range(per):
+            m = i * per + j
+            cm = int(counts[m])
+            same &= cm == int(c[j]) and torch.equal(outs[m][0][:cm], fr[j, :cm]) and \
+                torch.equal(outs[m][1][:cm], va[j, :cm])
+    torch.cuda.synchronize()
+    return {"plans": len(keys), "signals": len(xs), "keys": [list(k_) for k_ in keys],
+            "ms_per_call": ms_mix, "transforms_per_s": len(xs) / ms_mix * 1e3,
+            "sum_of_groups_alone_ms": ms_alone, "identical_to_alone": bool(same),
+            "form": "sfft_exec_many_plans, one group of kernels per plan on one stream "
+                    "(FP32+refine, SNR 30 dB)"}
+
+
 def other_configs(torch, dev):
     """The reference's other workloads, measured beside the headline:
     plain v1 (n=2^20, k=50, noiseless) and v2 (n=2^22, k=100, SNR 30 dB)
@@ -415,6 +485,8 @@ def other_configs(torch, dev):
                      if same else None}
         del x, d, Xf, got
         torch.cuda.empty_cache()
+    out["v12_uneven_mix"] = uneven_v12_mix(torch, dev)
+    torch.cuda.empty_cache()
     # batched v3, uneven k in one launch
     n, S = 1 << 24, 64
     ks = [200, 500, 1000]

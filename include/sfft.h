@@ -144,7 +144,9 @@ int sfft_generate_input(int n, int k, long drand48_seed, double std_noise, compl
  * signal i runs plans[i]; each plan draws for its signals in their order
  * here.  The v3 signals of all plans run as one launch (one workgroup per
  * signal, each reading its own plan's tables), so a batch can mix n and k;
- * v1/v2 signals run as one batch per plan.  in[i], out[i].freq and
+ * that launch goes on the stream of the first v3 plan, after the work
+ * enqueued so far on the other v3 plans' streams.  v1/v2 signals run as
+ * one batch per plan, on that plan's stream.  in[i], out[i].freq and
  * out[i].val must be device pointers, with out[i].capacity at least
  * sfft_plan_output_capacity(plans[i]).  Returns 0 or -1. */
 int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *const *in,
@@ -220,7 +222,9 @@ int sfft_derive_params(int n, int k, sfft_version version, int *p10);
 /* v3 outcome flags of the signals of the plan's last exec call, in signal
  * order (for sfft_exec_many_plans: this plan's signals in call order): bit 0
  * the reference's early return (src/computefourier-3.0.cc:859-860, empty
- * result), bit 1 result map overflow (the call failed), bit 2 the
+ * result), bit 1 result map overflow (the call returns -1 after writing
+ * every signal's count and result, this signal's truncated to the map's
+ * capacity), bit 2 the
  * estimate/subtract loop stopped at its 3*4096-pass cap instead of at
  * repeated bucket counts, bit 3 the refine form kept the reference's values
  * in a bucket it could not solve.  Copies up to num flags; returns the number of
@@ -257,6 +261,16 @@ int64_t sfft_debug_buckets(const sfft_plan *plan, int signal, complex_t *dst,
  * batch (at most cap of them) and returns how many there are, or -1. */
 int sfft_plan_debug_comb_votes(sfft_plan *plan, int enabled);
 int64_t sfft_debug_comb_votes(const sfft_plan *plan, int signal, int32_t *hits, int64_t cap);
+
+/* Diagnostics of a v3 plan's later execs.  serial_subtractions != 0: every
+ * subtraction pass (src/computefourier-3.0.cc:465-482, applied in map
+ * order) by one thread per bucket array, the reference's serial order,
+ * instead of the owner-thread paths used above 16 pairs; the results must
+ * be bit-identical either way (the tests compare them).  map_limit > 0:
+ * the map of located frequencies holds at most that many entries (below
+ * sfft_plan_output_capacity), to exercise the overflow handling; 0 keeps
+ * the plan's capacity.  Returns 0, or -1 for a bad or non-v3 plan. */
+int sfft_plan_debug_v3(sfft_plan *plan, int serial_subtractions, int map_limit);
 
 /* The plan's filters (which: 0 = location / first Gauss filter, 1 =
  * estimation / second Gauss filter): its w taps into time (capacity

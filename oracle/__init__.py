@@ -206,7 +206,7 @@ class V12:
 
     def __init__(self, n, k, with_comb):
         self._h = lib().or_v12_plan_make(n, k, int(with_comb))
-        self.p = self._h.contents.p
+        self.p = V12Params.from_buffer_copy(self._h.contents.p)  # outlives the plan
         self.n = self.p.n
         self.loops = self.p.loc_loops + self.p.est_loops
 
@@ -283,7 +283,7 @@ class V3:
 
     def __init__(self, n, k, bare=False):
         self._h = (lib().or_v3_plan_make_bare if bare else lib().or_v3_plan_make)(n, k)
-        self.p = self._h.contents.p
+        self.p = V3Params.from_buffer_copy(self._h.contents.p)  # outlives the plan
         self.n = self.p.n
 
     def __del__(self):

@@ -61,6 +61,15 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
   }
   const int chunk = chunk_signals(P, host_in);
   hipStream_t st = P->stream;
+  // on an error, host dense outputs not yet delivered read as zero (the
+  // reference zero-fills before anything else, src/sfft.cc:121-123)
+  auto fail = [&](int from) {
+    if (dense)
+      for (int s = from; s < num; s++)
+        if (dense[s] && !kinds.is_device(dense[s])) std::memset(dense[s], 0, (size_t)n * 16);
+    return -1;
+  };
+  bool partial = false;  // a v3 map overflowed: results written, error set
   for (int c0 = 0; c0 < num; c0 += chunk) {
     const int S = std::min(chunk, num - c0);
     std::vector<const double2 *> ins(S);
@@ -71,7 +80,7 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
     // inputs
     int staged = 0;
     for (int s = 0; s < S; s++) staged += !in_dev[c0 + s];
-    if (staged && !P->ws->stage.ensure((size_t)staged * n * 16)) return -1;
+    if (staged && !P->ws->stage.ensure((size_t)staged * n * 16)) return fail(c0);
     int slot = 0;
     for (int s = 0; s < S; s++) {
       const complex_t *src = in[c0 + s];
@@ -81,7 +90,7 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
         double2 *dst = P->ws->stage.as<double2>() + (size_t)slot++ * n;
         if (hipMemcpyAsync(dst, src, n * 16, hipMemcpyHostToDevice, st) != hipSuccess) {
           set_error("input copy failed");
-          return -1;
+          return fail(c0);
         }
         ins[s] = dst;
       }
@@ -93,11 +102,11 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
         complex_t *o = dense[c0 + s];
         if (!o) {
           set_error("null output signal");
-          return -1;
+          return fail(c0);
         }
         if (kinds.is_device(o)) {
           de[s] = reinterpret_cast<double2 *>(o);
-          if (k::zero_dense(de[s], n, st) != 0) return -1;
+          if (k::zero_dense(de[s], n, st) != 0) return fail(c0);
         } else {
           via_ws[s] = 1;
           nws++;
@@ -117,7 +126,7 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
     if (nws) {
       if (!P->ws->out_freq.ensure((size_t)nws * cap * 4) ||
           !P->ws->out_val.ensure((size_t)nws * cap * 16))
-        return -1;
+        return fail(c0);
       int q = 0;
       for (int s = 0; s < S; s++) {
         if (!via_ws[s]) continue;
@@ -135,7 +144,9 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
     io.dense = dense ? de.data() : nullptr;
     io.cap = cap;
     io.counts = counts.data();
-    if (exec_batch(P, io) != 0) return -1;
+    const int rc = exec_batch(P, io);
+    if (rc < 0) return fail(c0);
+    partial |= rc > 0;
     // results that went through the plan's buffers
     for (int s = 0; s < S; s++) {
       const int64_t cnt = std::min(counts[s], cap);
@@ -146,7 +157,7 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
         if (cnt && (hipMemcpy(hf.data(), fr[s], cnt * 4, hipMemcpyDeviceToHost) != hipSuccess ||
                     hipMemcpy(hv.data(), va[s], cnt * 16, hipMemcpyDeviceToHost) != hipSuccess)) {
           set_error("result copy failed");
-          return -1;
+          return fail(c0 + s);
         }
         double2 *o = reinterpret_cast<double2 *>(dense[c0 + s]);
         std::memset(o, 0, (size_t)n * 16);  // src/sfft.cc:121-123
@@ -169,7 +180,7 @@ int run(Plan *P, int num, const complex_t *const *in, complex_t *const *dense,
       }
     }
   }
-  return 0;
+  return partial ? -1 : 0;
 }
 
 }  // namespace
@@ -397,13 +408,38 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
       return -1;
     }
   }
-  if (!v3plans.empty() &&
-      exec_multi_v3(v3plans.data(), v3ios.data(), (int)v3plans.size(), v3plans[0]->stream,
-                    v3plans[0]->ws) != 0)
-    return -1;
+  int v3rc = 0;
+  if (!v3plans.empty()) {
+    // one launch on the first v3 plan's stream; it first waits for the work
+    // the caller enqueued on the other v3 plans' streams (their inputs)
+    hipStream_t st = v3plans[0]->stream;
+    std::vector<hipStream_t> waited;
+    for (Plan *P : v3plans) {
+      if (P->stream == st || std::find(waited.begin(), waited.end(), P->stream) != waited.end())
+        continue;
+      waited.push_back(P->stream);
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        set_error("sfft_exec_many_plans: hipEventCreate failed");
+        return -1;
+      }
+      const bool ok = hipEventRecord(ev, P->stream) == hipSuccess &&
+                      hipStreamWaitEvent(st, ev, 0) == hipSuccess;
+      (void)hipEventDestroy(ev);  // released once the wait has passed
+      if (!ok) {
+        set_error("sfft_exec_many_plans: stream ordering failed");
+        return -1;
+      }
+    }
+    v3rc = exec_multi_v3(v3plans.data(), v3ios.data(), (int)v3plans.size(), st,
+                         v3plans[0]->ws);
+    if (v3rc < 0) return -1;
+  }
+  // with a v3 map overflow (v3rc 1) every count and result is written, the
+  // overflowed signals' truncated (status bit 1), and the call fails
   for (size_t g = 0; g < G; g++)
     for (size_t j = 0; j < members[g].size(); j++) out[members[g][j]].count = counts[g][j];
-  return 0;
+  return v3rc ? -1 : 0;
 }
 
 int sfft_plan_set_schedule(sfft_plan *plan, int fold_mode, int stream_blocks)
@@ -601,6 +637,24 @@ int sfft_plan_debug_comb_votes(sfft_plan *plan, int enabled)
   }
   auto lk = hold(P);
   P->debug_votes = enabled != 0;
+  return 0;
+}
+
+int sfft_plan_debug_v3(sfft_plan *plan, int serial_subtractions, int map_limit)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if (P->version != SFFT_VERSION_3) {
+    set_error("the v3 diagnostics apply to v3 plans only");
+    return -1;
+  }
+  if (map_limit < 0) {
+    set_error("map_limit must be >= 0");
+    return -1;
+  }
+  auto lk = hold(P);
+  P->debug_v3_serial = serial_subtractions != 0;
+  P->debug_v3_map_limit = map_limit;
   return 0;
 }
 

@@ -659,6 +659,9 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   w += b_ph;
   a.ms_loops = ms_loops;
   a.refine = refine ? 1 : 0;
+  a.serial_all = P->debug_v3_serial ? 1 : 0;
+  a.map_limit = P->debug_v3_map_limit > 0 ? std::min(P->debug_v3_map_limit, P->v3_Mcap)
+                                          : P->v3_Mcap;
   a.ref_ms = refine && !big_ms ? reinterpret_cast<double2 *>(w) : nullptr;
   {
     // refine offset stride: odd, near 0.618 n/W, below n/W
@@ -696,7 +699,9 @@ static int v3_collect(const V3Pending &pd, int S, hipStream_t st)
   return 0;
 }
 
-// After the stream has drained.
+// After the stream has drained.  Returns 1 when some signal's map
+// overflowed (error set; every signal's count and result are written, the
+// overflowed ones truncated, status bit 1).
 static int v3_finish(Plan *P, BatchIO &io, const V3Pending &pd)
 {
   const int S = io.S;
@@ -706,7 +711,7 @@ static int v3_finish(Plan *P, BatchIO &io, const V3Pending &pd)
   for (int s = 0; s < S; s++)
     if (pd.h_status[s] & 2) {
       set_error("v3 result map overflow (more located frequencies than the plan holds)");
-      return -1;
+      return 1;
     }
   return 0;
 }
@@ -770,9 +775,9 @@ int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Works
   for (int g = 0; g < G; g++)
     if (v3_collect(pd[g], ios[g].S, st) != 0) return -1;
   HIP_OK(hipStreamSynchronize(st));
-  int rc = 0;
+  int rc = 0;  // 1: some map overflowed (every result written)
   for (int g = 0; g < G; g++)
-    if (v3_finish(plans[g], ios[g], pd[g]) != 0) rc = -1;
+    if (v3_finish(plans[g], ios[g], pd[g]) != 0) rc = 1;
   return rc;
 }
 

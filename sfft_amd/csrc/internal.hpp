@@ -190,6 +190,8 @@ struct Plan {
   bool last_rows_layout = false;  // the last batch stored buckets by alias row
   bool ref_conj = true;
   bool debug_votes = false;  // v2 FP64: count the comb votes (diagnostic)
+  bool debug_v3_serial = false;  // v3: every subtraction pass by one thread per array
+  int debug_v3_map_limit = 0;     // v3: map capacity below Mcap (0: Mcap)
   int64_t votes_stride = 0;  // int64 per signal in ws->votes of the last batch
   Rng rng;
   hipStream_t stream = nullptr;
@@ -373,6 +375,8 @@ struct V3Launch {
   // the reference places its false positives; ref_tinv = ref_t^-1 mod n/W.
   int refine;
   int ref_t, ref_tinv;
+  int serial_all;            // every subtraction pass serial (diagnostic)
+  int map_limit;             // map entries allowed (Mcap, or lower for diagnostics)
   double2 *ref_ms;
   int *map_keys;             // Mcap (insertion order)
   double2 *map_vals;         // Mcap

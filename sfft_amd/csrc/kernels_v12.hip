@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(64 * kFoldWaves) k_fold_stream(V12Launch a, Fo
 // summation order; with one chunk it is plain ascending tap order.
 constexpr int kResPer = 16;                        // residues per workgroup
 constexpr int kFoldResBlock = kResPer * kMaxLoops;  // 512: one pair per thread
-constexpr int kChunkBatch = 8;                     // gathers in flight per thread and round
+constexpr int kChunkBatch = 16;                    // gathers in flight per thread and round
 
 struct FoldChunks {
   int l0, nl;        // loops [l0, l0 + nl), all with B buckets

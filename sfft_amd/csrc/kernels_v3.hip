@@ -331,7 +331,7 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
   // One thread per bucket, each taking the pairs in pair order: a bucket
   // sees the same subtractions in the same order as when one thread applies
   // every pair in turn (cf3 465-482), so the results are the same.
-  if (cnt <= kSerialApply) {
+  if (cnt <= kSerialApply || c.a->serial_all) {
     // few pairs: one thread per bucket array applies them all
     if (threadIdx.x == 0 && (which & 1)) {
       for (int i = 0; i < cnt; i++) gauss_apply_pair(c.GS0, c.GS1, c.B1, c.remi[i * 3], c.rem + (int64_t)i * 14);
@@ -529,7 +529,7 @@ __device__ __forceinline__ bool map_add_found(const Ctx &c, int F, int *sh, int 
   int total;
   int pos = block_exclusive_scan(nnew, sh, total);
   const int base = *s_size;
-  if (base + total > c.a->Mcap) return false;
+  if (base + total > c.a->map_limit) return false;
   for (int i = lo; i < hi; i++) {
     int e = c.remi[i * 3];
     if (e < 0) {

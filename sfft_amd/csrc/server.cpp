@@ -356,14 +356,19 @@ void worker_main(sfft_server *S, Worker *W)
       io.cap = clones[g]->output_capacity();
       io.counts = counts[g].data();
     }
+    for (Plan *c : clones) c->last_status.clear();
     const int rc = multi ? exec_multi_v3(clones.data(), ios.data(), (int)G, W->stream, &W->ws)
                          : exec_batch(clones[0], ios[0]);
     const std::string err = rc == 0 ? "" : std::string(last_error());
 
     lk.lock();
     for (size_t g = 0; g < G; g++) {
-      for (size_t i = 0; i < grp[g].size(); i++)
-        finish(S, grp[g][i], rc == 0, counts[g][i], err);
+      // rc 1: some v3 map overflowed; the other requests' results are whole
+      const std::vector<int> &stat = clones[g]->last_status;
+      for (size_t i = 0; i < grp[g].size(); i++) {
+        const bool ok = rc == 0 || (rc == 1 && i < stat.size() && !(stat[i] & 2));
+        finish(S, grp[g][i], ok, counts[g][i], ok ? std::string() : err);
+      }
       S->plans[keys[g]].users--;
     }
     W->reserved = 0;

@@ -180,6 +180,11 @@ class sfft:
         self._L.sfft_plan_memory(self.sfft_plan, C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def set_stream(self, stream):
+        """Run this plan's execs on a torch.cuda.Stream (sfft_plan_set_stream)."""
+        import torch
+        self._bind_stream(torch, stream.cuda_stream)
+
     def _bind_stream(self, torch, s=None):
         if s is None:
             s = torch.cuda.current_stream().cuda_stream
@@ -290,6 +295,13 @@ class sfft:
             raise SfftError(_lib.last_error())
         return buf[:m].copy()
 
+    def debug_v3(self, serial_subtractions=False, map_limit=0):
+        """v3 plans (diagnostics): every subtraction pass serial, and/or the
+        map of located frequencies capped at map_limit entries."""
+        if self._L.sfft_plan_debug_v3(self.sfft_plan, int(bool(serial_subtractions)),
+                                      int(map_limit)) != 0:
+            raise SfftError(_lib.last_error())
+
     def debug_comb_votes(self, enabled=True):
         """v2 FP64 plans: count the location votes in later execs."""
         if self._L.sfft_plan_debug_comb_votes(self.sfft_plan, int(bool(enabled))) != 0:
@@ -321,19 +333,22 @@ class sfft:
         return fr[:m].copy(), va[:m].copy()
 
 
-def execute_many_plans(plans, xs, outputs=None, trim=True):
+def execute_many_plans(plans, xs, outputs=None, trim=True, bind_streams=True):
     """sfft_exec_many_plans: signal i of xs (device tensors) runs plans[i];
     the v3 signals of all plans run as one launch.  outputs: list of
     (freq, val) device tensors (allocated at each plan's capacity when
-    None).  Returns [(freq[:count], val[:count])], or with trim=False the
-    counts (int64 array) the call wrote, leaving the views to the caller."""
+    None).  Every plan runs on torch's current stream, or with
+    bind_streams=False on the stream it was given (set_stream).  Returns
+    [(freq[:count], val[:count])], or with trim=False the counts (int64
+    array) the call wrote, leaving the views to the caller."""
     import torch
     num = len(xs)
     assert len(plans) == num
     stream = torch.cuda.current_stream().cuda_stream
     addr = {}
     for p in {id(p): p for p in plans}.values():
-        p._bind_stream(torch, stream)
+        if bind_streams:
+            p._bind_stream(torch, stream)
         addr[id(p)] = C.cast(p.sfft_plan, C.c_void_p).value
     if outputs is None:
         outputs = [(torch.empty(p.capacity, dtype=torch.int32, device=x.device),

@@ -208,3 +208,111 @@ def test_v3_refine_values(torch_cuda, n, k):
               f"{e0.max(initial=0):.3g} refined {e1.max(initial=0):.3g}", flush=True)
         assert not status[i] & 8                 # every bucket solved
         assert e1.max(initial=0) <= 1e-6
+
+
+def test_many_plans_waits_for_other_plans_streams(torch_cuda):
+    """sfft_exec_many_plans launches the v3 signals of all plans on the first
+    v3 plan's stream, after the work already enqueued on the other v3
+    plans' streams: the second plan's input is written on its own stream
+    behind a ~50 ms GPU spin, and the results equal those of ready inputs."""
+    torch = torch_cuda
+    from sfft_amd.sfft import execute_many_plans
+    n = 1 << 18
+    x1, _, _ = signal_np(n, 50, seed=8101)
+    x2, _, _ = signal_np(n, 100, seed=8102)
+    d1 = torch.from_numpy(x1).cuda()
+    src2 = torch.from_numpy(x2).cuda()
+    d2 = torch.zeros_like(src2)
+    p1 = sfft_amd.sfft(n, 50, 3, seed=(5, 6))
+    p2 = sfft_amd.sfft(n, 100, 3, seed=(7, 8))
+    want = execute_many_plans([p1, p2], [d1, src2])
+    torch.cuda.synchronize()
+    assert all(f.numel() > 0 for f, _ in want)
+    p1.seed(5, 6)
+    p2.seed(7, 8)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    p1.set_stream(s1)
+    p2.set_stream(s2)
+    print("case v3 many-plans stream order", flush=True)
+    with torch.cuda.stream(s2):
+        torch.cuda._sleep(100_000_000)
+        d2.copy_(src2)
+    got = execute_many_plans([p1, p2], [d1, d2], bind_streams=False)
+    torch.cuda.synchronize()
+    for (fw, vw), (fg, vg) in zip(want, got):
+        assert torch.equal(fw, fg) and torch.equal(vw, vg)
+
+
+@pytest.mark.parametrize("n,k", [(1 << 24, 1000), (1 << 22, 2000), (1 << 20, 300)])
+def test_v3_owner_apply_equals_serial(torch_cuda, n, k):
+    """The owner-thread subtraction paths (B <= 512: each thread holds its
+    bucket in registers; B > 512: 32-pair masks; passes above 128 pairs
+    staged in LDS in chunks of 256) give each bucket the same subtractions
+    in the same order as one thread applying every pair in turn
+    (src/computefourier-3.0.cc:465-482): outputs bit-identical with
+    sfft_plan_debug_v3_serial on and off.  B_g1 = 512 / 1024 / 256 and
+    B_g2 = 128 / 256 / 64 here, and the whole-map passes (~k pairs) are not
+    multiples of 32 and cross the 256-pair chunks."""
+    torch = torch_cuda
+    print(f"case v3 owner vs serial n={n} k={k}", flush=True)
+    S = 4
+    xd = [torch.from_numpy(signal_np(n, k, seed=9100 + 7 * i + k)[0]).cuda() for i in range(S)]
+    a = sfft_amd.sfft(n, k, 3, seed=(41, 43))
+    b = sfft_amd.sfft(n, k, 3, seed=(41, 43))
+    b.debug_v3(serial_subtractions=True)
+    fa, va = a.make_sparse_outputs(S)
+    fb, vb = b.make_sparse_outputs(S)
+    ca = a.execute_many_sparse(xd, fa, va)
+    cb = b.execute_many_sparse(xd, fb, vb)
+    torch.cuda.synchronize()
+    assert np.array_equal(ca, cb) and int(ca.min()) > 0
+    for i in range(S):
+        c = int(ca[i])
+        assert torch.equal(fa[i, :c], fb[i, :c])
+        assert torch.equal(va[i, :c], vb[i, :c])
+
+
+def test_v3_map_overflow_writes_every_result(torch_cuda):
+    """A v3 map overflow, forced from the host (sfft_plan_debug_v3 caps the
+    map at 12 entries), fails the call but writes every signal's count and
+    result: the signal that fits equals an uncapped run, the overflowed one
+    has status bit 1 and at most 12 entries, and sfft_exec's host dense
+    output is zero outside the located entries."""
+    import ctypes as C
+    from sfft_amd import _lib
+    torch = torch_cuda
+    n, k = 1 << 16, 10
+    small = signal_np(n, 4, seed=1201)[0]
+    big = signal_np(n, 30, seed=1202)[0]
+    xd = [torch.from_numpy(small).cuda(), torch.from_numpy(big).cuda()]
+    ref = sfft_amd.sfft(n, k, 3, seed=(3, 4))
+    capped = sfft_amd.sfft(n, k, 3, seed=(3, 4))
+    capped.debug_v3(map_limit=12)
+    fr, va = ref.make_sparse_outputs(2)
+    c_ref = ref.execute_many_sparse(xd, fr, va)
+    torch.cuda.synchronize()
+    assert 0 < int(c_ref[0]) <= 12 < int(c_ref[1])
+    print("case v3 map overflow, sparse", flush=True)
+    fr2, va2 = capped.make_sparse_outputs(2)
+    outs = np.zeros((2, 4), dtype=np.uint64)
+    outs[:, 1] = fr2.shape[1]
+    outs[:, 2] = [fr2[i].data_ptr() for i in range(2)]
+    outs[:, 3] = [va2[i].data_ptr() for i in range(2)]
+    ins = np.array([x.data_ptr() for x in xd], dtype=np.uint64)
+    capped._bind_stream(torch)
+    r = capped._L.sfft_exec_many_sparse(capped.sfft_plan, 2, ins.ctypes.data_as(C.c_void_p),
+                                        outs.ctypes.data_as(C.POINTER(_lib.SfftOutput)))
+    torch.cuda.synchronize()
+    assert r == -1 and "overflow" in _lib.last_error()
+    st = capped.last_status()
+    assert not st[0] & 2 and st[1] & 2
+    c0 = int(outs[0, 0])
+    assert c0 == int(c_ref[0])
+    assert torch.equal(fr2[0, :c0], fr[0, :c0]) and torch.equal(va2[0, :c0], va[0, :c0])
+    assert 0 <= int(outs[1, 0]) <= 12
+    print("case v3 map overflow, host dense", flush=True)
+    dense = np.full(n, np.nan, dtype=np.complex128)
+    capped._L.sfft_exec(capped.sfft_plan, big.ctypes.data_as(C.c_void_p),
+                        dense.ctypes.data_as(C.c_void_p))
+    assert "overflow" in _lib.last_error()
+    assert not np.isnan(dense).any() and np.count_nonzero(dense) <= 12
