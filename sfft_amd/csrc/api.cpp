@@ -387,6 +387,14 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
   std::vector<BatchIO> ios(G);
   std::vector<Plan *> v3plans;
   std::vector<BatchIO> v3ios;
+  // v1/v2 groups are enqueued one after another without waiting (each on
+  // its plan's stream, from its plan's staging), then waited for together
+  std::vector<V12Pending> pend(G);
+  std::vector<size_t> v12g;
+  auto drain = [&]() {  // on an error: nothing of this call left in flight
+    for (size_t g : v12g) (void)hipStreamSynchronize(order[g]->stream);
+    return -1;
+  };
   for (size_t g = 0; g < G; g++) {
     for (int i : members[g]) {
       ins[g].push_back(reinterpret_cast<const double2 *>(in[i]));
@@ -404,8 +412,9 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
     if (order[g]->version == SFFT_VERSION_3) {
       v3plans.push_back(order[g]);
       v3ios.push_back(io);
-    } else if (exec_batch(order[g], io) != 0) {
-      return -1;
+    } else {
+      if (exec_batch_async(order[g], io, pend[g]) != 0) return drain();
+      v12g.push_back(g);
     }
   }
   int v3rc = 0;
@@ -421,20 +430,22 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
       hipEvent_t ev = nullptr;
       if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
         set_error("sfft_exec_many_plans: hipEventCreate failed");
-        return -1;
+        return drain();
       }
       const bool ok = hipEventRecord(ev, P->stream) == hipSuccess &&
                       hipStreamWaitEvent(st, ev, 0) == hipSuccess;
       (void)hipEventDestroy(ev);  // released once the wait has passed
       if (!ok) {
         set_error("sfft_exec_many_plans: stream ordering failed");
-        return -1;
+        return drain();
       }
     }
     v3rc = exec_multi_v3(v3plans.data(), v3ios.data(), (int)v3plans.size(), st,
                          v3plans[0]->ws);
-    if (v3rc < 0) return -1;
+    if (v3rc < 0) return drain();
   }
+  for (size_t g : v12g)
+    if (exec_batch_wait(order[g], ios[g], pend[g]) != 0) return drain();
   // with a v3 map overflow (v3rc 1) every count and result is written, the
   // overflowed signals' truncated (status bit 1), and the call fails
   for (size_t g = 0; g < G; g++)

@@ -795,9 +795,33 @@ static uint64_t workspace_signature(const Workspace &w)
   return h;
 }
 
-int exec_batch(Plan *P, BatchIO &io)
+// After the stream work of a v1/v2 batch: wait, stage times, counts.
+static int v12_finish(Plan *P, BatchIO &io, const V12Pending &q)
 {
-  if (P->version == SFFT_VERSION_3) return exec_batch_v3(P, io);
+  HIP_OK(hipStreamSynchronize(q.st));
+  if (q.timing) {
+    for (int i = 0; i < 5; i++) {
+      // only stages that ran: an event never recorded fails the query, and
+      // that error would be reported by the next launch check
+      if (!q.launches[i]) continue;
+      float ms = 0;
+      const hipError_t e = i == 0 ? hipEventElapsedTime(&ms, P->ev_comb[0], P->ev_comb[1])
+                                  : hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]);
+      if (e == hipSuccess) {
+        P->stage_ms[i] += ms;
+        P->stage_launches[i] += q.launches[i];
+      }
+    }
+  }
+  if (io.counts)
+    for (int s = 0; s < q.S; s++) io.counts[s] = q.h_counts[s];
+  return 0;
+}
+
+// A v1/v2 batch: draws, uploads and stream work; with pd, returns once the
+// work is enqueued (exec_batch_wait finishes it), else waits and finishes.
+static int exec_batch_v12(Plan *P, BatchIO &io, V12Pending *pd)
+{
   const int S = io.S;
   const V12Params &p = P->v12;
   const int L = p.loops(), cl = p.comb ? p.comb_loops : 0;
@@ -1041,24 +1065,34 @@ int exec_batch(Plan *P, BatchIO &io)
   } else if (enqueue(timing) != 0) {
     return -1;
   }
-  HIP_OK(hipStreamSynchronize(st));
-  if (timing) {
-    for (int i = 0; i < 5; i++) {
-      // only stages that ran: an event never recorded fails the query, and
-      // that error would be reported by the next launch check
-      if (!launches[i]) continue;
-      float ms = 0;
-      const hipError_t e = i == 0 ? hipEventElapsedTime(&ms, P->ev_comb[0], P->ev_comb[1])
-                                  : hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]);
-      if (launches[i] && e == hipSuccess) {
-        P->stage_ms[i] += ms;
-        P->stage_launches[i] += launches[i];
-      }
-    }
+  V12Pending local;
+  V12Pending &q = pd ? *pd : local;
+  q.st = st;
+  q.timing = timing;
+  q.S = S;
+  q.h_counts = h_counts;
+  std::copy(launches, launches + 5, q.launches);
+  return pd ? 0 : v12_finish(P, io, q);
+}
+
+int exec_batch(Plan *P, BatchIO &io)
+{
+  if (P->version == SFFT_VERSION_3) return exec_batch_v3(P, io);
+  return exec_batch_v12(P, io, nullptr);
+}
+
+int exec_batch_async(Plan *P, BatchIO &io, V12Pending &pd)
+{
+  if (P->version == SFFT_VERSION_3) {
+    set_error("exec_batch_async: v1/v2 plans only");
+    return -1;
   }
-  if (io.counts)
-    for (int s = 0; s < S; s++) io.counts[s] = h_counts[s];
-  return 0;
+  return exec_batch_v12(P, io, &pd);
+}
+
+int exec_batch_wait(Plan *P, BatchIO &io, const V12Pending &pd)
+{
+  return v12_finish(P, io, pd);
 }
 
 }  // namespace sfa

@@ -284,11 +284,26 @@ struct BatchIO {
 };
 
 // Draws the RNG values of S signals in signal order, runs the kernels on
-// the plan's stream and waits for them.  Returns 0 or -1 (set_error).
+// the plan's stream and waits for them.  Returns 0, -1 (set_error), or for
+// v3 1 when a signal's map overflowed (every result written, error set).
 int exec_batch(Plan *P, BatchIO &io);
+// A v1/v2 batch in two halves: exec_batch_async draws and enqueues the
+// stream work (uploads from the plan's pinned staging, kernels, the count
+// copy back) and returns; exec_batch_wait waits for the plan's stream and
+// delivers the counts.  Batches of different plans can be in flight
+// together; the plan must not run anything else in between.
+struct V12Pending {
+  hipStream_t st = nullptr;
+  bool timing = false;
+  int S = 0;
+  int launches[5] = {0, 0, 0, 0, 0};
+  const int64_t *h_counts = nullptr;  // in the plan's pinned staging
+};
+int exec_batch_async(Plan *P, BatchIO &io, V12Pending &pd);
+int exec_batch_wait(Plan *P, BatchIO &io, const V12Pending &pd);
 // v3 batches of G plans (different n, k) as one launch on stream st; the
 // descriptors live in ws (not any plan's).  Plans' own workspaces hold their
-// signals' state.  Returns 0 or -1.
+// signals' state.  Returns 0, -1, or 1 (a map overflowed, results written).
 int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws);
 
 // ---------------------------------------------------------------- kernels

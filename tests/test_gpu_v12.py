@@ -384,3 +384,39 @@ def test_v2_comb_votes_match_oracle(torch_cuda, n, k, snr):
     assert np.array_equal(got, appended)
     # every appended frequency duplicates a prefill entry
     assert np.all(np.isin(appended, ref["hits"][:ref["hits_prefill"]]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_many_plans_v12_mix_equals_alone(torch_cuda, precision):
+    """sfft_exec_many_plans over v1/v2 plans of different (n, k), plus a v3
+    plan, interleaved: the v1/v2 groups are enqueued back to back and
+    waited for together, and every signal equals its plan's alone run
+    (each plan draws for its own signals in order)."""
+    torch = torch_cuda
+    from sfft_amd.sfft import execute_many_plans
+    keys = [(2, 1 << 18, 50), (1, 1 << 18, 20), (2, 1 << 20, 100), (3, 1 << 18, 100),
+            (2, 1 << 16, 10)]
+    order = [0, 1, 2, 3, 4, 2, 1, 0, 4, 3, 2]
+    xs = [torch.from_numpy(signal_np(keys[g][1], keys[g][2], seed=700 + i,
+                                     snr_db=None if keys[g][0] == 3 else 30.0)[0]).cuda()
+          for i, g in enumerate(order)]
+
+    def plan(g):
+        v, n, k = keys[g]
+        return sfft_amd.sfft(n, k, v, strict=False, precision=precision,
+                             seed=(61 + g, 4242 + g))
+    plans = [plan(g) for g in range(len(keys))]
+    res = execute_many_plans([plans[g] for g in order], xs)
+    torch.cuda.synchronize()
+    for g, (v, n, k) in enumerate(keys):
+        alone = plan(g)
+        idx = [i for i, gg in enumerate(order) if gg == g]
+        fr, va = alone.make_sparse_outputs(len(idx))
+        counts = alone.execute_many_sparse([xs[i] for i in idx], fr, va)
+        torch.cuda.synchronize()
+        for j, i in enumerate(idx):
+            print(f"case many-plans {precision} v{v} n={n} k={k} signal {i}", flush=True)
+            c = int(counts[j])
+            assert c > 0 and int(res[i][0].numel()) == c
+            assert torch.equal(res[i][0], fr[j, :c])
+            assert torch.equal(res[i][1], va[j, :c])
