@@ -251,7 +251,7 @@ int64_t Plan::bytes_per_signal() const
     const int64_t off = ref ? kV3RefineOffsets : 0;
     const int64_t ms = 32LL + (big ? 32LL * (ref ? off + 1 : 2) : 16LL * off);
     return ms * v3.W_Man + (int64_t)v3_Mcap * 28 + (int64_t)v3_Rcap * 236 +
-           (int64_t)v3_Fcap * 40 + 16LL * (v3.w_g2 + 2) + 512;
+           (int64_t)v3_Fcap * 40 + 16LL * (v3.w_g2 + 2) + 32LL * (v3.B_g1 + v3.B_g2) + 768;
   }
   const V12Params &p = v12;
   const bool fp32 = precision == SFFT_PRECISION_FP32_REFINE;
@@ -597,8 +597,9 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   const int ms_loops = refine && big_ms ? kV3RefineOffsets + 1 : 2;
   const size_t b_pre = big_ms ? al(Sz * ms_loops * q.W_Man * 16) : 0;
   const size_t b_ref = refine && !big_ms ? al(Sz * kV3RefineOffsets * q.W_Man * 16) : 0;
+  const size_t b_gpre = al(Sz * (2 * (size_t)q.B_g1 + 2 * (size_t)q.B_g2) * 16);
   const size_t total = b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi +
-                       b_ph + 2 * b_pre + b_ref;
+                       b_ph + 2 * b_pre + b_ref + b_gpre;
   if (!P->ws->params.ensure(param_bytes) || !P->ws->ptrs.ensure(4 * tab_bytes) ||
       !P->ws->v3.ensure(total) || !P->ws->count.ensure(cnt_bytes) ||
       !P->ws->v3_status.ensure((size_t)S * 4))
@@ -663,6 +664,9 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   a.map_limit = P->debug_v3_map_limit > 0 ? std::min(P->debug_v3_map_limit, P->v3_Mcap)
                                           : P->v3_Mcap;
   a.ref_ms = refine && !big_ms ? reinterpret_cast<double2 *>(w) : nullptr;
+  w += b_ref;
+  a.gpre = reinterpret_cast<double2 *>(w);
+  w += b_gpre;
   {
     // refine offset stride: odd, near 0.618 n/W, below n/W
     const int np = n / q.W_Man;

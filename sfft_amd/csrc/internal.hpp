@@ -388,6 +388,7 @@ struct V3Launch {
   // ref_t is odd, near 0.618 n/W: members of a bucket m W apart sit at
   // phase steps m ref_t / (n/W) mod 1, well apart for the small m at which
   // the reference places its false positives; ref_tinv = ref_t^-1 mod n/W.
+  double2 *gpre;             // Gauss buckets from k_v3_pre: S x (2 B1 + 2 B2)
   int refine;
   int ref_t, ref_tinv;
   int serial_all;            // every subtraction pass serial (diagnostic)

@@ -652,156 +652,126 @@ __device__ double ref_solve(const double2 (&e)[kRefMax][kRefOff], const double2 
   return res;
 }
 
-// The refine pass of one signal after its loop (all threads; contains
-// barriers).  M located entries in c.keys / c.vals (insertion order).
-__device__ void refine_values(const Ctx &c, const double2 *x, int M, double2 *lds, int *s_flag)
+// Row l of signal s's refine buckets: the offset g0 + l t (rows 0, 2, 3,
+// ... of the Mansour pre-pass above the LDS size, whose row 1 is the
+// reference's g0 + 1; else ref_ms, written by k_v3_pre).
+__device__ __forceinline__ const double2 *ref_row(const V3Launch &a, int s, int l)
 {
-  const V3Launch &a = *c.a;
-  const int n = c.n, W = c.W, t = a.ref_t, tinv = a.ref_tinv;
+  return a.ms_pre ? a.ms_pre + ((int64_t)s * a.ms_loops + (l ? l + 1 : 0)) * a.W
+                  : a.ref_ms + ((int64_t)s * kRefOff + l) * a.W;
+}
+
+// The refine solve of bucket j of signal s (one thread): members from the
+// lists head/next (entries of keys/vals), values written to vals.  Returns
+// 1 when the bucket keeps the reference's values.
+__device__ int refine_bucket(const V3Launch &a, int s, int g0, int j, const int *head,
+                             const int *next, const int *keys, double2 *vals)
+{
+  const int n = a.n, W = a.W, t = a.ref_t, tinv = a.ref_tinv;
   const unsigned mask = (unsigned)n - 1;
-  // row l: the buckets at offset g0 + l t (rows 0, 2, 3, ... of the
-  // pre-pass, whose row 1 is the reference's g0 + 1)
-  auto row = [&](int l) -> const double2 * {
-    return a.ms_pre ? a.ms_pre + ((int64_t)c.s * a.ms_loops + (l ? l + 1 : 0)) * W
-                    : a.ref_ms + ((int64_t)c.s * kRefOff + l) * W;
-  };
-  if (!a.ms_pre) {
-    const int sigma = n / W, logW = ilog2v(W);
-    double2 *RM = a.ref_ms + (int64_t)c.s * kRefOff * W;  // [l][j]
-    for (int l = 0; l < kRefOff; l++) {
-      const unsigned off = (unsigned)c.g0 + (unsigned)l * (unsigned)t;
-      for (int i = threadIdx.x; i < W; i += blockDim.x)
-        lds[bitrev((unsigned)i, logW)] = x[(off + (unsigned)i * (unsigned)sigma) & mask];
-      __syncthreads();
-      lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
-      for (int i = threadIdx.x; i < W; i += blockDim.x) RM[l * W + i] = lds[i];
-      __syncthreads();
+  int mem[kRefMax];
+  int p = 0;
+  for (int e = head[j]; e >= 0; e = next[e]) {
+    if (p == kRefMax) {
+      p = kRefMax + 1;
+      break;
     }
+    mem[p++] = e;
   }
-  // the located entries of every residue class: linked lists through
-  // head (W entries, in cf) and next (M entries, in remi)
-  int *head = c.cf, *next = c.remi;
-  for (int j = threadIdx.x; j < W; j += blockDim.x) head[j] = -1;
-  __threadfence_block();
-  __syncthreads();
-  for (int e = threadIdx.x; e < M; e += blockDim.x)
-    next[e] = atomicExch(&head[c.keys[e] & (W - 1)], e);
-  __threadfence_block();
-  __syncthreads();
-  int kept = 0;
-  for (int j = threadIdx.x; j < W; j += blockDim.x) {
-    int mem[kRefMax];
-    int p = 0;
-    for (int e = head[j]; e >= 0; e = next[e]) {
-      if (p == kRefMax) {
-        p = kRefMax + 1;
-        break;
-      }
-      mem[p++] = e;
+  if (p == 0) return 0;
+  if (p > kRefMax || p >= kRefOff) return 1;  // no spare equation: keep the reference values
+  // members by frequency (the list order depends on the atomics)
+  for (int u = 1; u < p; u++)
+    for (int v = u; v > 0 && keys[mem[v]] < keys[mem[v - 1]]; v--) {
+      const int tmp = mem[v];
+      mem[v] = mem[v - 1];
+      mem[v - 1] = tmp;
     }
-    if (p == 0) continue;
-    if (p > kRefMax || p >= kRefOff) {  // no spare equation: keep the reference values
-      kept = 1;
-      continue;
-    }
-    // members by frequency (the list order depends on the atomics)
-    for (int u = 1; u < p; u++)
-      for (int v = u; v > 0 && c.keys[mem[v]] < c.keys[mem[v - 1]]; v--) {
-        const int tmp = mem[v];
-        mem[v] = mem[v - 1];
-        mem[v - 1] = tmp;
-      }
-    double2 y[kRefOff], e[kRefMax][kRefOff], X[kRefMax], w[kRefMax];
-    double scale = 0;
-    for (int l = 0; l < kRefOff; l++) {
-      const double2 m = row(l)[j];
-      y[l] = make_double2(m.x / W, m.y / W);
-      scale += y[l].x * y[l].x + y[l].y * y[l].y;
-    }
-    auto column = [&](int m, long long f) {
-      for (int l = 0; l < kRefOff; l++)
-        e[m][l] = cis_mod((f * ((long long)c.g0 + (long long)l * t)) & mask, n);
-    };
+  double2 y[kRefOff], e[kRefMax][kRefOff], X[kRefMax], w[kRefMax];
+  double scale = 0;
+  for (int l = 0; l < kRefOff; l++) {
+    const double2 m = ref_row(a, s, l)[j];
+    y[l] = make_double2(m.x / W, m.y / W);
+    scale += y[l].x * y[l].x + y[l].y * y[l].y;
+  }
+  auto column = [&](int m, long long f) {
+    for (int l = 0; l < kRefOff; l++)
+      e[m][l] = cis_mod((f * ((long long)g0 + (long long)l * t)) & mask, n);
+  };
+  for (int m = 0; m < p; m++) {
+    const long long f = keys[mem[m]];
+    column(m, f);
+    w[m] = cis_mod((f * t) & mask, n);
+  }
+  // residual at most 1e-10 of the bucket (or of 1e-4, below which a
+  // bucket holds rounding only)
+  const double tol = 1e-20 * (scale > 1e-8 ? scale : 1e-8);
+  double res = ref_solve(e, y, p, X);
+  if (res > tol) {
+    // annihilate the members' nodes: d_l <- d_{l+1} - w_m d_l, once per member
+    double2 d[kRefOff];
+    int len = kRefOff;
+    for (int l = 0; l < kRefOff; l++) d[l] = y[l];
     for (int m = 0; m < p; m++) {
-      const long long f = c.keys[mem[m]];
-      column(m, f);
-      w[m] = cis_mod((f * t) & mask, n);
+      for (int l = 0; l + 1 < len; l++) d[l] = csub(d[l + 1], cmul(w[m], d[l]));
+      len--;
     }
-    // residual at most 1e-10 of the bucket (or of 1e-4, below which a
-    // bucket holds rounding only)
-    const double tol = 1e-20 * (scale > 1e-8 ? scale : 1e-8);
-    double res = ref_solve(e, y, p, X);
-    if (res > tol) {
-      // annihilate the members' nodes: d_l <- d_{l+1} - w_m d_l, once per member
-      double2 d[kRefOff];
-      int len = kRefOff;
-      for (int l = 0; l < kRefOff; l++) d[l] = y[l];
-      for (int m = 0; m < p; m++) {
-        for (int l = 0; l + 1 < len; l++) d[l] = csub(d[l + 1], cmul(w[m], d[l]));
-        len--;
+    double2 X2[kRefMax];
+    if (p + 1 <= kRefMax && len >= 2) {  // one unlocated tone: w_u = d_{l+1} / d_l
+      double2 num = make_double2(0, 0);
+      double den = 0;
+      for (int l = 0; l + 1 < len; l++) {
+        num = cadd(num, cconjmul(d[l], d[l + 1]));
+        den += d[l].x * d[l].x + d[l].y * d[l].y;
       }
-      double2 X2[kRefMax];
-      if (p + 1 <= kRefMax && len >= 2) {  // one unlocated tone: w_u = d_{l+1} / d_l
-        double2 num = make_double2(0, 0);
-        double den = 0;
-        for (int l = 0; l + 1 < len; l++) {
-          num = cadd(num, cconjmul(d[l], d[l + 1]));
-          den += d[l].x * d[l].x + d[l].y * d[l].y;
+      if (den > 0) {
+        column(p, ref_snap(atan2(num.y, num.x), j, n, W, t, tinv));
+        const double res2 = ref_solve(e, y, p + 1, X2);
+        if (res2 <= tol) {
+          res = res2;
+          for (int m = 0; m < p; m++) X[m] = X2[m];
         }
-        if (den > 0) {
-          column(p, ref_snap(atan2(num.y, num.x), j, n, W, t, tinv));
-          const double res2 = ref_solve(e, y, p + 1, X2);
+      }
+    }
+    if (res > tol && p + 2 <= kRefMax && len >= 4) {
+      // two unlocated tones: least-squares predictor over l < len - 2,
+      // normal equations of the columns (d_{l+1}, d_l) against -d_{l+2}
+      double g11 = 0, g22 = 0;
+      double2 g12 = make_double2(0, 0), b1 = make_double2(0, 0), b2 = make_double2(0, 0);
+      for (int l = 0; l + 2 < len; l++) {
+        g11 += d[l + 1].x * d[l + 1].x + d[l + 1].y * d[l + 1].y;
+        g22 += d[l].x * d[l].x + d[l].y * d[l].y;
+        g12 = cadd(g12, cconjmul(d[l + 1], d[l]));
+        b1 = csub(b1, cconjmul(d[l + 1], d[l + 2]));
+        b2 = csub(b2, cconjmul(d[l], d[l + 2]));
+      }
+      const double det = g11 * g22 - (g12.x * g12.x + g12.y * g12.y);
+      if (det > 0) {
+        const double2 g21 = make_double2(g12.x, -g12.y);
+        const double2 n1 = csub(make_double2(g22 * b1.x, g22 * b1.y), cmul(g12, b2));
+        const double2 n0 = csub(make_double2(g11 * b2.x, g11 * b2.y), cmul(g21, b1));
+        const double2 c1 = make_double2(n1.x / det, n1.y / det);
+        const double2 c0 = make_double2(n0.x / det, n0.y / det);
+        const double2 sq = csqrt_d(csub(cmul(c1, c1), make_double2(4 * c0.x, 4 * c0.y)));
+        const double2 z1 = make_double2(0.5 * (sq.x - c1.x), 0.5 * (sq.y - c1.y));
+        const double2 z2 = make_double2(-0.5 * (sq.x + c1.x), -0.5 * (sq.y + c1.y));
+        const long long u1 = ref_snap(atan2(z1.y, z1.x), j, n, W, t, tinv);
+        const long long u2 = ref_snap(atan2(z2.y, z2.x), j, n, W, t, tinv);
+        if (u1 != u2) {
+          column(p, u1);
+          column(p + 1, u2);
+          const double res2 = ref_solve(e, y, p + 2, X2);
           if (res2 <= tol) {
             res = res2;
             for (int m = 0; m < p; m++) X[m] = X2[m];
           }
         }
       }
-      if (res > tol && p + 2 <= kRefMax && len >= 4) {
-        // two unlocated tones: least-squares predictor over l < len - 2,
-        // normal equations of the columns (d_{l+1}, d_l) against -d_{l+2}
-        double g11 = 0, g22 = 0;
-        double2 g12 = make_double2(0, 0), b1 = make_double2(0, 0), b2 = make_double2(0, 0);
-        for (int l = 0; l + 2 < len; l++) {
-          g11 += d[l + 1].x * d[l + 1].x + d[l + 1].y * d[l + 1].y;
-          g22 += d[l].x * d[l].x + d[l].y * d[l].y;
-          g12 = cadd(g12, cconjmul(d[l + 1], d[l]));
-          b1 = csub(b1, cconjmul(d[l + 1], d[l + 2]));
-          b2 = csub(b2, cconjmul(d[l], d[l + 2]));
-        }
-        const double det = g11 * g22 - (g12.x * g12.x + g12.y * g12.y);
-        if (det > 0) {
-          const double2 g21 = make_double2(g12.x, -g12.y);
-          const double2 n1 = csub(make_double2(g22 * b1.x, g22 * b1.y), cmul(g12, b2));
-          const double2 n0 = csub(make_double2(g11 * b2.x, g11 * b2.y), cmul(g21, b1));
-          const double2 c1 = make_double2(n1.x / det, n1.y / det);
-          const double2 c0 = make_double2(n0.x / det, n0.y / det);
-          const double2 sq = csqrt_d(csub(cmul(c1, c1), make_double2(4 * c0.x, 4 * c0.y)));
-          const double2 z1 = make_double2(0.5 * (sq.x - c1.x), 0.5 * (sq.y - c1.y));
-          const double2 z2 = make_double2(-0.5 * (sq.x + c1.x), -0.5 * (sq.y + c1.y));
-          const long long u1 = ref_snap(atan2(z1.y, z1.x), j, n, W, t, tinv);
-          const long long u2 = ref_snap(atan2(z2.y, z2.x), j, n, W, t, tinv);
-          if (u1 != u2) {
-            column(p, u1);
-            column(p + 1, u2);
-            const double res2 = ref_solve(e, y, p + 2, X2);
-            if (res2 <= tol) {
-              res = res2;
-              for (int m = 0; m < p; m++) X[m] = X2[m];
-            }
-          }
-        }
-      }
-    }
-    if (res <= tol) {
-      for (int m = 0; m < p; m++) c.vals[mem[m]] = X[m];
-    } else {
-      kept = 1;
     }
   }
-  if (kept) atomicOr(s_flag, 8);
-  __threadfence_block();
-  __syncthreads();
+  if (res > tol) return 1;
+  for (int m = 0; m < p; m++) vals[mem[m]] = X[m];
+  return 0;
 }
 
 __device__ void write_result(const V3Launch &a, int s, const int *keys, const double2 *vals,
@@ -840,9 +810,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   c.ev = a.ev + (int64_t)s * a.Fcap;
   c.rem = a.rem + (int64_t)s * a.Rcap * 14;
   c.remi = a.remi + (int64_t)s * a.Rcap * 3;
-  const int n = a.n, W = a.W;
-  const unsigned mask = (unsigned)n - 1;
-  const double2 *x = a.x[s];
+  const int W = a.W;
   for (int i = threadIdx.x; i < a.Hcap; i += blockDim.x) c.hidx[i] = -1;
   if (threadIdx.x == 0) {
     s_size = 0;
@@ -851,24 +819,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   __threadfence_block();
   __syncthreads();
 
-  // ---- Mansour filter (cf3 96-127): two aliased sub-samplings, W-pt FFTs
-  if (a.ms_pre) {
-    const double2 *pre = a.ms_pre + (int64_t)s * a.ms_loops * W;
-    for (int i = threadIdx.x; i < 2 * W; i += blockDim.x) c.MS[2 * (i % W) + i / W] = pre[i];
-  } else {
-    const int sigma = n / W, logW = ilog2v(W);
-    for (int l = 0; l < 2; l++) {
-      for (int i = threadIdx.x; i < W; i += blockDim.x)
-        lds[bitrev((unsigned)i, logW)] =
-            x[((unsigned)c.g0 + (unsigned)l + (unsigned)i * (unsigned)sigma) & mask];
-      __syncthreads();
-      lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
-      for (int i = threadIdx.x; i < W; i += blockDim.x) c.MS[2 * i + l] = lds[i];
-      __syncthreads();
-    }
-  }
-  __threadfence_block();
-  __syncthreads();
+  // ---- Mansour filter (cf3 96-127): buckets from k_v3_pre
   int F = est_mansour(c, sh);
   if (!map_add_found(c, F, sh, &s_size)) {
     if (threadIdx.x == 0) s_flag |= 2;
@@ -888,26 +839,14 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
     return;
   }
 
-  // ---- Gauss filter 1 (cf3 130-212): contiguous taps folded into B1
+  // ---- Gauss filter 1 (cf3 130-212): buckets from k_v3_pre
   {
-    const int B = a.B1, chunks = a.w1 / B;
-    for (int r = threadIdx.x; r < B; r += blockDim.x) {
-      double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
-      for (int ch = 0; ch < chunks; ch++) {
-        const int t = ch * B + r;
-        const double2 tp = a.taps1[t];
-        const double2 x0 = x[((unsigned)c.gG + (unsigned)t) & mask];
-        const double2 x1 = x[((unsigned)c.gG + 1u + (unsigned)t) & mask];
-        s0.x = s0.x + (tp.x * x0.x - tp.y * x0.y);
-        s0.y = s0.y + (tp.y * x0.x + tp.x * x0.y);
-        s1.x = s1.x + (tp.x * x1.x - tp.y * x1.y);
-        s1.y = s1.y + (tp.y * x1.x + tp.x * x1.y);
-      }
-      c.GS0[r] = s0;
-      c.GS1[r] = s1;
+    const double2 *gp = a.gpre + (int64_t)s * (2 * a.B1 + 2 * a.B2);
+    for (int r = threadIdx.x; r < a.B1; r += blockDim.x) {
+      c.GS0[r] = gp[r];
+      c.GS1[r] = gp[a.B1 + r];
     }
     __syncthreads();
-    fft_pair(c.GS0, c.GS1, B, a);
   }
   subtract(c, c.keys, c.vals, s_size, 1);  // the map so far, insertion order
   F = est_gauss(c, c.GS0, c.GS1, a.B1, 1, 0, a.win1, a.win1_half, sh);
@@ -915,58 +854,14 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   __syncthreads();
   subtract(c, c.ef, c.ev, F, 1 | 4);
 
-  // ---- Gauss filter 2 (cf3 215-297): permuted samples times e^{2 pi i b t/n}
+  // ---- Gauss filter 2 (cf3 215-297): buckets from k_v3_pre
   {
-    // permute_fourier (cf3 66-87) forms the phase of sample i as
-    // shift * step^i by repeated complex multiplication; one lane replays
-    // that chain so the samples carry the reference's rounding.
-    // The chain goes to LDS past the four Gauss bucket arrays when it fits
-    // (a lone lane's global stores made it several times slower), else to
-    // the signal's global scratch.
-    const size_t ph_off = 2 * (size_t)a.B1 + 2 * (size_t)a.B2;
-    double2 *phase = (ph_off + a.w2 + 1) * sizeof(double2) <= a.lds_bytes
-                         ? lds + ph_off
-                         : a.phase + (int64_t)s * a.Pcap;
-    if (threadIdx.x == 0) {
-      double s0, c0, s1, c1;
-      sincos(2 * M_PI * c.gG * c.b / n, &s0, &c0);
-      sincos(2 * M_PI * c.ai * c.b / n, &s1, &c1);
-      double2 sh = make_double2(c0, s0);
-      const double2 st = make_double2(c1, s1);
-      // unrolled: the loop control and stores stay off the product chain
-#pragma unroll 8
-      for (int i = 0; i <= a.w2; i++) {
-        phase[i] = sh;
-        sh = make_double2(sh.x * st.x - sh.y * st.y, sh.x * st.y + sh.y * st.x);
-      }
-    }
-    __threadfence_block();
-    __syncthreads();
-    const int B = a.B2, chunks = a.w2 / B;
-    for (int r = threadIdx.x; r < B; r += blockDim.x) {
-      double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
-      for (int ch = 0; ch < chunks; ch++) {
-        const int t = ch * B + r;
-        const double2 tp = a.taps2[t];
-        double2 p[2];
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-          // sample (gG + (t+q) ai) mod n (cf3 78-85)
-          const unsigned idx =
-              (unsigned)(((long long)c.gG + (long long)(t + q) * c.ai) % n);
-          const double2 xv = x[idx], sh = phase[t + q];
-          p[q] = make_double2(xv.x * sh.x - xv.y * sh.y, xv.x * sh.y + xv.y * sh.x);
-        }
-        s0.x = s0.x + (tp.x * p[0].x - tp.y * p[0].y);
-        s0.y = s0.y + (tp.y * p[0].x + tp.x * p[0].y);
-        s1.x = s1.x + (tp.x * p[1].x - tp.y * p[1].y);
-        s1.y = s1.y + (tp.y * p[1].x + tp.x * p[1].y);
-      }
-      c.GP0[r] = s0;
-      c.GP1[r] = s1;
+    const double2 *gp = a.gpre + (int64_t)s * (2 * a.B1 + 2 * a.B2) + 2 * a.B1;
+    for (int r = threadIdx.x; r < a.B2; r += blockDim.x) {
+      c.GP0[r] = gp[r];
+      c.GP1[r] = gp[a.B2 + r];
     }
     __syncthreads();
-    fft_pair(c.GP0, c.GP1, B, a);
   }
   subtract(c, c.keys, c.vals, s_size, 2);
   F = est_gauss(c, c.GP0, c.GP1, a.B2, c.aa, c.b, a.win2, a.win2_half, sh);
@@ -1020,7 +915,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   }
   if (pass >= a.max_passes && threadIdx.x == 0) s_flag |= 4;
   __syncthreads();
-  if (a.refine) {  // k_v3_refine refines the map and writes the result
+  if (a.refine) {  // the refine kernels refine the map and write the result
     if (threadIdx.x == 0) {
       a.count[s] = s_size;
       a.status[s] = s_flag;
@@ -1080,6 +975,148 @@ __device__ void write_result(const V3Launch &a, int s, const int *keys, const do
   }
 }
 
+// The start of every signal's transform that does not depend on its map of
+// located frequencies, batch-wide, four workgroups per signal (the batch's
+// other CUs would idle while one workgroup per signal did it in turn):
+// block ntask j + t of (group, signal) blocks[j] computes
+//   t = 0, 1: Mansour loop t (cf3 96-127): the sub-samples x[g0 + t + i n/W]
+//             and their W-point FFT into the signal's interleaved buckets
+//             (with the pre-pass above W = 8192, block 0 copies them);
+//   t = 2:    Gauss filter 1 (cf3 130-212), both offsets, folded and FFT'd;
+//   t = 3:    Gauss filter 2 (cf3 215-297): permute_fourier's phase chain
+//             (cf3 66-87), the permuted fold and the FFTs;
+//   t = 4 + l (ntask = 4 + kRefOff when a group refines): the refine form's
+//             buckets at offset g0 + l ref_t.
+// Same operations in the same order as when k_v3_multi computed them, so
+// the buckets are the same bits.  The Gauss buckets go to gpre
+// (GS0 | GS1 | GP0 | GP1 per signal).
+__global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict__ groups,
+                                                     const int2 *__restrict__ blocks, int ntask)
+{
+  extern __shared__ double2 lds[];
+  const int2 bb = blocks[blockIdx.x / (unsigned)ntask];
+  const int task = (int)(blockIdx.x % (unsigned)ntask);
+  const V3Launch &a = groups[bb.x];
+  const int s = bb.y;
+  const int n = a.n, W = a.W;
+  const unsigned mask = (unsigned)n - 1;
+  const int *prm = a.params + (int64_t)s * 6;
+  const int ai = prm[1], b = prm[2], g0 = prm[4], gG = prm[5];
+  const double2 *x = a.x[s];
+  double2 *gp = a.gpre + (int64_t)s * (2 * a.B1 + 2 * a.B2);
+  if (task >= 4) {
+    // the refine form's buckets at g0 + l t, l = task - 4 (from the
+    // Mansour pre-pass instead above the LDS size)
+    if (!a.refine || a.ms_pre) return;
+    const int l = task - 4, sigma = n / W, logW = ilog2v(W);
+    const unsigned off = (unsigned)g0 + (unsigned)l * (unsigned)a.ref_t;
+    for (int i = threadIdx.x; i < W; i += blockDim.x)
+      lds[bitrev((unsigned)i, logW)] = x[(off + (unsigned)i * (unsigned)sigma) & mask];
+    __syncthreads();
+    lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
+    double2 *RM = a.ref_ms + ((int64_t)s * kRefOff + l) * W;
+    for (int i = threadIdx.x; i < W; i += blockDim.x) RM[i] = lds[i];
+    return;
+  }
+  if (task < 2) {
+    double2 *MS = a.ms + (int64_t)s * 2 * W;
+    if (a.ms_pre) {
+      if (task == 0) {
+        const double2 *pre = a.ms_pre + (int64_t)s * a.ms_loops * W;
+        for (int i = threadIdx.x; i < 2 * W; i += blockDim.x) MS[2 * (i % W) + i / W] = pre[i];
+      }
+      return;
+    }
+    const int l = task, sigma = n / W, logW = ilog2v(W);
+    for (int i = threadIdx.x; i < W; i += blockDim.x)
+      lds[bitrev((unsigned)i, logW)] =
+          x[((unsigned)g0 + (unsigned)l + (unsigned)i * (unsigned)sigma) & mask];
+    __syncthreads();
+    lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
+    for (int i = threadIdx.x; i < W; i += blockDim.x) MS[2 * i + l] = lds[i];
+    return;
+  }
+  if (task == 2) {
+    const int B = a.B1, chunks = a.w1 / B;
+    double2 *GS0 = lds, *GS1 = lds + B;
+    for (int r = threadIdx.x; r < B; r += blockDim.x) {
+      double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
+      for (int ch = 0; ch < chunks; ch++) {
+        const int t = ch * B + r;
+        const double2 tp = a.taps1[t];
+        const double2 x0 = x[((unsigned)gG + (unsigned)t) & mask];
+        const double2 x1 = x[((unsigned)gG + 1u + (unsigned)t) & mask];
+        s0.x = s0.x + (tp.x * x0.x - tp.y * x0.y);
+        s0.y = s0.y + (tp.y * x0.x + tp.x * x0.y);
+        s1.x = s1.x + (tp.x * x1.x - tp.y * x1.y);
+        s1.y = s1.y + (tp.y * x1.x + tp.x * x1.y);
+      }
+      GS0[r] = s0;
+      GS1[r] = s1;
+    }
+    __syncthreads();
+    fft_pair(GS0, GS1, B, a);
+    __syncthreads();
+    for (int r = threadIdx.x; r < B; r += blockDim.x) {
+      gp[r] = GS0[r];
+      gp[B + r] = GS1[r];
+    }
+    return;
+  }
+  // task 3: permute_fourier forms the phase of sample i as shift * step^i
+  // by repeated complex multiplication; one lane replays that chain so the
+  // samples carry the reference's rounding (into LDS past the bucket arrays
+  // when it fits: a lone lane's global stores are several times slower)
+  const int B = a.B2, chunks = a.w2 / B;
+  double2 *GP0 = lds, *GP1 = lds + B;
+  double2 *phase = (2 * (size_t)B + a.w2 + 1) * sizeof(double2) <= a.lds_bytes
+                       ? lds + 2 * B
+                       : a.phase + (int64_t)s * a.Pcap;
+  if (threadIdx.x == 0) {
+    double s0, c0, s1, c1;
+    sincos(2 * M_PI * gG * b / n, &s0, &c0);
+    sincos(2 * M_PI * ai * b / n, &s1, &c1);
+    double2 sh = make_double2(c0, s0);
+    const double2 st = make_double2(c1, s1);
+    // unrolled: the loop control and stores stay off the product chain
+#pragma unroll 8
+    for (int i = 0; i <= a.w2; i++) {
+      phase[i] = sh;
+      sh = make_double2(sh.x * st.x - sh.y * st.y, sh.x * st.y + sh.y * st.x);
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int r = threadIdx.x; r < B; r += blockDim.x) {
+    double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
+    for (int ch = 0; ch < chunks; ch++) {
+      const int t = ch * B + r;
+      const double2 tp = a.taps2[t];
+      double2 p[2];
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        // sample (gG + (t+q) ai) mod n (cf3 78-85)
+        const unsigned idx = (unsigned)(((long long)gG + (long long)(t + q) * ai) % n);
+        const double2 xv = x[idx], sv = phase[t + q];
+        p[q] = make_double2(xv.x * sv.x - xv.y * sv.y, xv.x * sv.y + xv.y * sv.x);
+      }
+      s0.x = s0.x + (tp.x * p[0].x - tp.y * p[0].y);
+      s0.y = s0.y + (tp.y * p[0].x + tp.x * p[0].y);
+      s1.x = s1.x + (tp.x * p[1].x - tp.y * p[1].y);
+      s1.y = s1.y + (tp.y * p[1].x + tp.x * p[1].y);
+    }
+    GP0[r] = s0;
+    GP1[r] = s1;
+  }
+  __syncthreads();
+  fft_pair(GP0, GP1, B, a);
+  __syncthreads();
+  for (int r = threadIdx.x; r < B; r += blockDim.x) {
+    gp[2 * a.B1 + r] = GP0[r];
+    gp[2 * a.B1 + B + r] = GP1[r];
+  }
+}
+
 // Signals of several plans (different n, k) in one launch: block b runs
 // signal blocks[b].y of plan group blocks[b].x.
 __global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *__restrict__ groups,
@@ -1091,35 +1128,57 @@ __global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *__restric
   v3_signal(groups[b.x], b.y);
 }
 
-// The refine form after k_v3_multi: the same (group, signal) blocks refine
-// their map's values (refine_values) and write the result.
+// The refine form after k_v3_multi, in three kernels over the same
+// (group, signal) blocks: k_v3_refine lists the located entries of every
+// residue class (linked lists through head, W entries in cf, and next, M
+// entries in remi); k_v3_refine_solve solves every bucket, one thread each,
+// nsub workgroups per signal; k_v3_refine_write sorts and writes the
+// result.  Groups without refine, and signals with the reference's early
+// return, wrote their result in k_v3_multi and are skipped.
 __global__ void __launch_bounds__(kV3Block) k_v3_refine(const V3Launch *__restrict__ groups,
                                                         const int2 *__restrict__ blocks)
 {
-  extern __shared__ double2 lds[];
-  __shared__ int s_flag;
   const int2 bb = blocks[blockIdx.x];
   const V3Launch &a = groups[bb.x];
-  if (!a.refine) return;  // written by k_v3_multi
   const int s = bb.y;
-  const int M = (int)a.count[s];
-  if (threadIdx.x == 0) s_flag = a.status[s];
+  if (!a.refine || (a.status[s] & 1)) return;
+  const int M = (int)a.count[s], W = a.W;
+  const int *keys = a.map_keys + (int64_t)s * a.Mcap;
+  int *head = a.cf + (int64_t)s * a.Fcap, *next = a.remi + (int64_t)s * a.Rcap * 3;
+  for (int j = threadIdx.x; j < W; j += blockDim.x) head[j] = -1;
+  __threadfence_block();
   __syncthreads();
-  if (s_flag & 1) return;  // the reference's early return: no result
-  Ctx c;
-  c.a = &a;
-  c.s = s;
-  c.n = a.n;
-  c.W = a.W;
-  c.B1 = a.B1;
-  c.B2 = a.B2;
-  c.g0 = a.params[(int64_t)s * 6 + 4];
-  c.keys = a.map_keys + (int64_t)s * a.Mcap;
-  c.vals = a.map_vals + (int64_t)s * a.Mcap;
-  c.cf = a.cf + (int64_t)s * a.Fcap;
-  c.remi = a.remi + (int64_t)s * a.Rcap * 3;
-  refine_values(c, a.x[s], M, lds, &s_flag);
-  write_result(a, s, c.keys, c.vals, M, s_flag, lds);
+  for (int e = threadIdx.x; e < M; e += blockDim.x)
+    next[e] = atomicExch(&head[keys[e] & (W - 1)], e);
+}
+
+__global__ void __launch_bounds__(kV3Block) k_v3_refine_solve(const V3Launch *__restrict__ groups,
+                                                              const int2 *__restrict__ blocks,
+                                                              int nsub)
+{
+  const int2 bb = blocks[blockIdx.x / (unsigned)nsub];
+  const int j = (int)(blockIdx.x % (unsigned)nsub) * blockDim.x + threadIdx.x;
+  const V3Launch &a = groups[bb.x];
+  const int s = bb.y;
+  if (!a.refine || j >= a.W || (a.status[s] & 1)) return;  // no barriers here
+  const int g0 = a.params[(int64_t)s * 6 + 4];
+  if (refine_bucket(a, s, g0, j, a.cf + (int64_t)s * a.Fcap, a.remi + (int64_t)s * a.Rcap * 3,
+                    a.map_keys + (int64_t)s * a.Mcap, a.map_vals + (int64_t)s * a.Mcap))
+    atomicOr(&a.status[s], 8);
+}
+
+__global__ void __launch_bounds__(kV3Block) k_v3_refine_write(const V3Launch *__restrict__ groups,
+                                                              const int2 *__restrict__ blocks)
+{
+  extern __shared__ double2 lds[];
+  const int2 bb = blocks[blockIdx.x];
+  const V3Launch &a = groups[bb.x];
+  const int s = bb.y;
+  if (!a.refine) return;
+  const int st = a.status[s];
+  if (st & 1) return;  // the reference's early return: no result
+  write_result(a, s, a.map_keys + (int64_t)s * a.Mcap, a.map_vals + (int64_t)s * a.Mcap,
+               (int)a.count[s], st, lds);
 }
 
 // Sub-samples of the Mansour loops for the pre-pass: ms_pre[(s*L+l)*W + i]
@@ -1176,20 +1235,38 @@ int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks
     set_error("v3: descriptor upload failed");
     return -1;
   }
+  bool refine = false;
+  int nsub = 1;
+  for (int g = 0; g < G; g++)
+    if (h_groups[g].refine) {
+      refine = true;
+      nsub = std::max(nsub, (h_groups[g].W + kV3Block - 1) / kV3Block);
+    }
+  const int ntask = refine ? 4 + kRefOff : 4;
+  allow_v3_lds(reinterpret_cast<const void *>(k_v3_pre), lds);
+  hipLaunchKernelGGL(k_v3_pre, dim3(ntask * nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks,
+                     ntask);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("v3 pre: ") + hipGetErrorString(e));
+    return -1;
+  }
   allow_v3_lds(reinterpret_cast<const void *>(k_v3_multi), lds);
   hipLaunchKernelGGL(k_v3_multi, dim3(nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks);
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("v3 multi: ") + hipGetErrorString(e));
     return -1;
   }
-  bool refine = false;
-  for (int g = 0; g < G; g++) refine |= h_groups[g].refine != 0;
   if (refine) {
     // groups without refine write their result in k_v3_multi; their blocks
-    // return at once here (status and count say so)
-    allow_v3_lds(reinterpret_cast<const void *>(k_v3_refine), lds);
-    hipLaunchKernelGGL(k_v3_refine, dim3(nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks);
+    // return at once here
+    hipLaunchKernelGGL(k_v3_refine, dim3(nblocks), dim3(kV3Block), 0, st, d_groups, d_blocks);
+    hipLaunchKernelGGL(k_v3_refine_solve, dim3(nblocks * nsub), dim3(kV3Block), 0, st, d_groups,
+                       d_blocks, nsub);
+    allow_v3_lds(reinterpret_cast<const void *>(k_v3_refine_write), lds);
+    hipLaunchKernelGGL(k_v3_refine_write, dim3(nblocks), dim3(kV3Block), lds, st, d_groups,
+                       d_blocks);
     e = hipGetLastError();
     if (e != hipSuccess) {
       set_error(std::string("v3 refine: ") + hipGetErrorString(e));
