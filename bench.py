@@ -576,7 +576,7 @@ def other_configs(torch, dev):
     out["v3_n24_b64_k200_500_1000_refined"] = {
         "transforms_per_s": S / dt_r, "ms_per_batch": dt_r * 1e3,
         "form": "fp64 location + values re-estimated by least squares over fresh Mansour "
-                "buckets at 6 strided offsets, integer-exact phases, Prony step for up to "
+                "buckets at 8 strided offsets, integer-exact phases, Prony step for up to "
                 "2 unlocated tones per bucket (precision fp32_refine)",
         "located_sets_equal_to_faithful": bool(same_sets),
         "max_value_error_recovered": err_r,

@@ -98,7 +98,7 @@ typedef enum sfft_precision {
    * comb selection and FP64 estimation of the located values.  v3: the
    * reference's FP64 location (FP32 phases cannot locate at n = 2^24), then
    * the located values re-estimated in FP64 by least squares over fresh
-   * Mansour buckets at 6 strided offsets with integer-exact phases, with up
+   * Mansour buckets at 8 strided offsets with integer-exact phases, with up
    * to two unlocated tones per bucket found by a Prony step (located set
    * unchanged; false positives come out near zero). */
   SFFT_PRECISION_FP32_REFINE = 1

@@ -23,7 +23,7 @@ constexpr int kMaxLdsFft64 = 8192;
 // Largest number of loops (location + estimation) the median network takes.
 constexpr int kMaxLoops = 32;
 // Mansour offsets of the v3 refine form (equations per bucket).
-constexpr int kV3RefineOffsets = 6;
+constexpr int kV3RefineOffsets = 8;
 
 void set_error(const std::string &msg);
 

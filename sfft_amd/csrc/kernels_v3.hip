@@ -574,7 +574,7 @@ __device__ __forceinline__ bool map_add_found(const Ctx &c, int F, int *sh, int 
 // the residue class and added as an unknown; if the residual is still large
 // the bucket keeps the reference's values (status bit 8).
 constexpr int kRefOff = kV3RefineOffsets;  // Mansour offsets: equations per bucket
-constexpr int kRefMax = 5;  // unknowns per bucket (located + unlocated)
+constexpr int kRefMax = 7;  // unknowns per bucket (located + unlocated)
 
 __device__ __forceinline__ double2 cis_mod(long long r, int n)
 {

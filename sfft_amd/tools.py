@@ -1,7 +1,7 @@
 """Command-line tools with the reference harnesses' flags.
 
     python -m sfft_amd.tools timing        [-n N] [-k K] [-r R] [-o] [-v V]
-    python -m sfft_amd.tools timing_many   [-n N] [-k K] [-i NUM] [-s] [-o] [-v V]
+    python -m sfft_amd.tools timing_many   [-n N] [-k K] [-i NUM] [-s] [-o] [-v V] [--check]
     python -m sfft_amd.tools verification  [-n N] [-k K] [-r R] [-o] [-v V]
 
 timing (src/timing.cc with src/simulation.cc:32-92) times R sfft_exec
@@ -85,10 +85,14 @@ def timing(argv):
 
 def timing_many(argv):
     args = _parser("timing_many", 1 << 18, 1 << 14, [
-        ("-i", dict(type=int, default=100)), ("-s", dict(action="store_true"))]).parse_args(argv)
+        ("-i", dict(type=int, default=100)), ("-s", dict(action="store_true")),
+        ("--check", dict(action="store_true",
+                         help="after timing, check every output as verification does"))
+    ]).parse_args(argv)
     plans = [_plan(args) for _ in range(args.i if args.s else 1)]
     seed = _seed(args)
-    xs = [reference_input(args.n, args.k, seed + t)[0] for t in range(args.i)]
+    ins = [reference_input(args.n, args.k, seed + t) for t in range(args.i)]
+    xs = [x for x, _, _ in ins]
     L = _lib.load()
     import ctypes as C
     if args.d:
@@ -111,7 +115,25 @@ def timing_many(argv):
     if args.d:
         torch.cuda.synchronize()
     print("TIME", time.perf_counter() - ts)
-    return 0
+    if not args.check:
+        return 0
+    bad = 0
+    for t, (_, X, _) in enumerate(ins):
+        ans = outs[t].cpu().numpy() if args.d else outs[t]
+        bad += _first_error(X, ans) is not None
+    print("CHECK %d of %d outputs OK" % (args.i - bad, args.i))
+    return 0 if bad == 0 else 3
+
+
+def _first_error(X, ans):
+    """verification's criterion (src/verification.cc:35-50): every input
+    frequency above 0.1 recovered within 0.1; None, or (i, code)."""
+    for i in np.nonzero(np.abs(X) > 0.1)[0]:
+        if ans[i] == 0:
+            return i, 1
+        if abs(ans[i] - X[i]) > 0.1:
+            return i, 2
+    return None
 
 
 def verification(argv):
@@ -121,14 +143,15 @@ def verification(argv):
     ans = None
     for _ in range(args.r):
         ans = plan.execute(x)
-    for i in np.nonzero(np.abs(X) > 0.1)[0]:
-        if ans[i] == 0:
+    err = _first_error(X, ans)
+    if err is not None:
+        i, code = err
+        if code == 1:
             print("ERROR: Frequency %d was not recovered!" % i)
-            return 1
-        if abs(ans[i] - X[i]) > 0.1:
+        else:
             print("ERROR: Error of frequency %d is too big:\n  Expected: %s\n  Actual  : %s"
                   % (i, X[i], ans[i]))
-            return 2
+        return code
     print("OK")
     return 0
 

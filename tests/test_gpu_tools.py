@@ -24,3 +24,15 @@ def test_timing_prints_time(torch_cuda, capsys):
     assert tools.main(["timing_many", "-n", "65536", "-k", "50", "-i", "4", "-s",
                        "--seed", "1"]) == 0
     assert capsys.readouterr().out.startswith("TIME ")
+
+
+@pytest.mark.parametrize("flags", [["-v", "1"], ["-v", "2", "-d"], ["-v", "3"],
+                                   ["-v", "2", "-s"], ["-v", "2", "-p", "fp32_refine"]])
+def test_timing_many_outputs_verified(torch_cuda, capsys, flags):
+    """timing_many's dense sfft_exec_many outputs (host arrays, device with
+    -d, one plan per input with -s), each checked with verification's
+    criterion against the generator's spectrum (src/verification.cc:35-50)."""
+    rc = tools.main(["timing_many", "-n", "65536", "-k", "50", "-i", "6", "--seed", "7",
+                     "--check"] + flags)
+    out = capsys.readouterr().out
+    assert rc == 0 and "CHECK 6 of 6 outputs OK" in out, out
