@@ -1,6 +1,7 @@
 // Plan construction (device-resident tables), RNG draws and batch execution
 // of sFFT v1/v2 on the GPU.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -511,9 +512,16 @@ void free_plan(Plan *P)
   }
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  for (hipEvent_t e : P->ev_part)
+    if (e) (void)hipEventDestroy(e);
+  if (P->ev_est) (void)hipEventDestroy(P->ev_est);
   if (P->stream2) {
     (void)hipStreamSynchronize(P->stream2);
     (void)hipStreamDestroy(P->stream2);
+  }
+  if (P->stream3) {
+    (void)hipStreamSynchronize(P->stream3);
+    (void)hipStreamDestroy(P->stream3);
   }
   if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
@@ -822,6 +830,31 @@ static int v12_finish(Plan *P, BatchIO &io, const V12Pending &q)
   return 0;
 }
 
+// Signals [base, base + cnt) of a batch's launch: every per-signal array
+// advanced by base signals (the kernels index them by signal alone).
+static k::V12Launch sub_launch(const k::V12Launch &a, int base, int cnt, int stride, int vchunks)
+{
+  k::V12Launch b = a;
+  const int64_t s0 = base;
+  b.S = cnt;
+  b.x = a.x + s0;
+  b.params = a.params + s0 * stride;
+  if (a.xs64) b.xs64 = a.xs64 + s0 * a.xs;
+  if (a.xs32) b.xs32 = a.xs32 + s0 * a.xs;
+  if (a.tmp64) b.tmp64 = a.tmp64 + s0 * a.xs;
+  if (a.tmp32) b.tmp32 = a.tmp32 + s0 * a.xs;
+  if (a.approved) b.approved = a.approved + s0 * a.comb_loops * a.num;
+  if (a.num_comb) b.num_comb = a.num_comb + s0;
+  if (a.J) b.J = a.J + s0 * a.Lloc * a.num;
+  if (a.hits) b.hits = a.hits + s0 * a.hits_cap;
+  if (a.vote_cnt) b.vote_cnt = a.vote_cnt + s0 * vchunks;
+  b.count = a.count + s0;
+  if (a.o_freq) b.o_freq = a.o_freq + s0;
+  if (a.o_val) b.o_val = a.o_val + s0;
+  if (a.o_dense) b.o_dense = a.o_dense + s0;
+  return b;
+}
+
 // A v1/v2 batch: draws, uploads and stream work; with pd, returns once the
 // work is enqueued (exec_batch_wait finishes it), else waits and finishes.
 static int exec_batch_v12(Plan *P, BatchIO &io, V12Pending *pd)
@@ -988,6 +1021,24 @@ static int exec_batch_v12(Plan *P, BatchIO &io, V12Pending *pd)
     HIP_OK(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
   }
   int launches[5] = {0, 0, 0, 0, 0};
+  // Large v2 FP32 batches (alias-row layout) run in parts: the fold and
+  // bucket FFTs of each part on the plan's stream, each part's estimation on
+  // stream3 once that part and the comb are done, so the estimation (FP64
+  // arithmetic) of part q overlaps the fold (memory latency) of part q+1.
+  // Every signal's kernels and inputs are the same, so results are too.
+  // Not with stage timing, whose stages are whole-batch launches.
+  // SFFT_PARTS overrides the part count (tuning experiments; 1: unparted).
+  int parts = 1;
+  if (two_streams && fp32 && a.rows_layout && !timing && !dbg_votes && S >= 64) {
+    parts = 2;
+    if (const char *e = std::getenv("SFFT_PARTS")) parts = std::atoi(e);
+    parts = std::max(1, std::min({parts, Plan::kMaxParts, S / 32}));
+  }
+  if (parts > 1 && !P->stream3) {
+    HIP_OK(hipStreamCreateWithFlags(&P->stream3, hipStreamNonBlocking));
+    for (hipEvent_t &e : P->ev_part) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&P->ev_est, hipEventDisableTiming));
+  }
   // The stream work of one batch: uploads, kernels, counts back.  The v2
   // comb (its own gathers and FFTs) runs on stream2 beside the fold.
   auto enqueue = [&](bool timed) -> int {
@@ -996,6 +1047,26 @@ static int exec_batch_v12(Plan *P, BatchIO &io, V12Pending *pd)
     };
     HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, param_bytes + 4 * tab_bytes,
                           hipMemcpyHostToDevice, st));
+    if (parts > 1) {
+      hipStream_t st2 = P->stream2, st3 = P->stream3;
+      HIP_OK(hipEventRecord(P->ev_fork, st));
+      HIP_OK(hipStreamWaitEvent(st2, P->ev_fork, 0));
+      if (k::comb(a, st2) != 0) return -1;
+      HIP_OK(hipEventRecord(P->ev_join, st2));
+      HIP_OK(hipStreamWaitEvent(st3, P->ev_join, 0));
+      for (int q = 0; q < parts; q++) {
+        const int b0 = (int)((int64_t)S * q / parts), b1 = (int)((int64_t)S * (q + 1) / parts);
+        const k::V12Launch sa = sub_launch(a, b0, b1 - b0, stride, vchunks);
+        if (k::fold_fft(sa, st, 0) < 0 || k::fold_fft(sa, st, 1) < 0) return -1;
+        HIP_OK(hipEventRecord(P->ev_part[q], st));
+        HIP_OK(hipStreamWaitEvent(st3, P->ev_part[q], 0));
+        if (k::estimate(sa, st3) != 0) return -1;
+      }
+      HIP_OK(hipEventRecord(P->ev_est, st3));
+      HIP_OK(hipStreamWaitEvent(st, P->ev_est, 0));
+      HIP_OK(hipMemcpyAsync(h_counts, a.count, cnt_bytes, hipMemcpyDeviceToHost, st));
+      return 0;
+    }
     if (two_streams) HIP_OK(hipEventRecord(P->ev_fork, st));
     mark(1);
     // the fold is enqueued before the comb: for a single signal the host's

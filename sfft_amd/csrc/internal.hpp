@@ -245,6 +245,11 @@ struct Plan {
   // fork after the uploads, join before estimation.
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Large v2 FP32 batches in parts: the fold and bucket FFTs of part q+1
+  // on the plan's stream while part q is estimated on stream3.
+  static constexpr int kMaxParts = 8;
+  hipStream_t stream3 = nullptr;
+  hipEvent_t ev_part[kMaxParts] = {}, ev_est = nullptr;
 
   int64_t output_capacity() const;
   int64_t plan_bytes() const;

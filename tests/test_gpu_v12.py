@@ -420,3 +420,34 @@ def test_many_plans_v12_mix_equals_alone(torch_cuda, precision):
             assert c > 0 and int(res[i][0].numel()) == c
             assert torch.equal(res[i][0], fr[j, :c])
             assert torch.equal(res[i][1], va[j, :c])
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_parted_batch_equals_single(torch_cuda, graphs):
+    """A v2 FP32 batch of 96 signals of n=2^22, k=64 runs in parts (the fold
+    and bucket FFTs of one part beside the estimation of the previous one,
+    on two streams, eagerly or replayed from a HIP graph); every signal gets
+    the bits it gets alone (a single signal runs unparted)."""
+    torch = torch_cuda
+    from sfft_amd.signals import signals_device
+    n, k, S = 1 << 22, 64, 96
+    print(f"case parted batch n={n} k={k} S={S} graphs={graphs}", flush=True)
+    x, _, _ = signals_device(n, k, list(range(700, 700 + S)), snr_db=40.0)
+    batch = _gpu_plan(2, n, k, "fp32_refine")
+    batch.enable_graphs(graphs)
+    fr, va = batch.make_sparse_outputs(S)
+    for rep in range(2 if graphs else 1):   # capture, then a replay
+        batch.seed(17, 4242)
+        counts = batch.execute_many_sparse(x, fr, va)
+        torch.cuda.synchronize()
+    single = _gpu_plan(2, n, k, "fp32_refine")
+    f1, v1 = single.make_sparse_outputs(1)
+    for i in range(S):
+        c1 = single.execute_many_sparse(x[i:i + 1], f1, v1)
+        torch.cuda.synchronize()
+        c = int(counts[i])
+        assert int(c1[0]) == c
+        assert torch.equal(fr[i, :c], f1[0, :c])
+        assert torch.equal(va[i, :c], v1[0, :c])
+    del x
+    torch.cuda.empty_cache()
