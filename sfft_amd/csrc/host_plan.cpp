@@ -552,23 +552,36 @@ struct V3Pending {
 
 // v3: draws (cf3 800-810) in signal order, the workspace layout and the
 // uploads of one batch (one workgroup per signal) on stream st.
-static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
+// Host and device staging of one v3 call: the groups' descriptors and
+// block list, each group's draws and pointer tables (uploaded with one
+// copy), then each group's counts and status flags (read back with one).
+struct V3Stage {
+  char *h = nullptr, *d = nullptr;  // pinned host / device, same offsets
+  size_t up = 0, dn = 0;            // next free offsets of the two regions
+};
+static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+static size_t v3_up_bytes(int S) { return al256((size_t)S * 6 * 4) + 4 * al256((size_t)S * 8); }
+static size_t v3_dn_bytes(int S) { return al256((size_t)S * 8) + al256((size_t)S * 4); }
+
+static int v3_prepare(Plan *P, BatchIO &io, V3Pending &pd, V3Stage &sg)
 {
   const int S = io.S;
   const V3Params &q = P->v3;
   const int n = q.n;
-  const size_t param_bytes = ((size_t)S * 6 * sizeof(int) + 255) & ~size_t(255);
-  const size_t tab_bytes = ((size_t)S * sizeof(void *) + 255) & ~size_t(255);
-  const size_t cnt_bytes = ((size_t)S * sizeof(int64_t) + 255) & ~size_t(255);
-  if (ensure_pinned(P, param_bytes + 4 * tab_bytes + cnt_bytes + (size_t)S * 4) != 0) return -1;
-  char *pin = reinterpret_cast<char *>(P->ws->pinned);
+  const size_t param_bytes = al256((size_t)S * 6 * sizeof(int));
+  const size_t tab_bytes = al256((size_t)S * sizeof(void *));
+  const size_t cnt_bytes = al256((size_t)S * sizeof(int64_t));
+  char *pin = sg.h + sg.up, *dup = sg.d + sg.up;
+  sg.up += param_bytes + 4 * tab_bytes;
+  char *pdn = sg.h + sg.dn, *ddn = sg.d + sg.dn;
+  sg.dn += cnt_bytes + al256((size_t)S * 4);
   int *hp = reinterpret_cast<int *>(pin);
   const void **t_in = reinterpret_cast<const void **>(pin + param_bytes);
   const void **t_freq = reinterpret_cast<const void **>(pin + param_bytes + tab_bytes);
   const void **t_val = reinterpret_cast<const void **>(pin + param_bytes + 2 * tab_bytes);
   const void **t_dense = reinterpret_cast<const void **>(pin + param_bytes + 3 * tab_bytes);
-  int64_t *h_counts = reinterpret_cast<int64_t *>(pin + param_bytes + 4 * tab_bytes);
-  int *h_status = reinterpret_cast<int *>(pin + param_bytes + 4 * tab_bytes + cnt_bytes);
+  int64_t *h_counts = reinterpret_cast<int64_t *>(pdn);
+  int *h_status = reinterpret_cast<int *>(pdn + cnt_bytes);
   for (int s = 0; s < S; s++) {
     int *d = hp + (size_t)s * 6;
     Rng &R = io.rng && io.rng[s] ? *io.rng[s] : P->rng;
@@ -608,17 +621,10 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   const size_t b_gpre = al(Sz * (2 * (size_t)q.B_g1 + 2 * (size_t)q.B_g2) * 16);
   const size_t total = b_ms + b_keys + b_vals + b_hidx + 2 * b_fi + 2 * b_fv + b_rem + b_remi +
                        b_ph + 2 * b_pre + b_ref + b_gpre;
-  if (!P->ws->params.ensure(param_bytes) || !P->ws->ptrs.ensure(4 * tab_bytes) ||
-      !P->ws->v3.ensure(total) || !P->ws->count.ensure(cnt_bytes) ||
-      !P->ws->v3_status.ensure((size_t)S * 4))
-    return -1;
+  if (!P->ws->v3.ensure(total)) return -1;
   pd.h_counts = h_counts;
   pd.h_status = h_status;
-  HIP_OK(hipMemcpyAsync(P->ws->params.p, hp, (size_t)S * 6 * sizeof(int),
-                        hipMemcpyHostToDevice, st));
-  HIP_OK(hipMemcpyAsync(P->ws->ptrs.p, pin + param_bytes, 4 * tab_bytes,
-                        hipMemcpyHostToDevice, st));
-  char *dtab = reinterpret_cast<char *>(P->ws->ptrs.p);
+  char *dtab = dup + param_bytes;
   char *w = reinterpret_cast<char *>(P->ws->v3.p);
   k::V3Launch &a = pd.a;
   a = k::V3Launch{};
@@ -631,7 +637,7 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   a.w1 = q.w_g1;
   a.w2 = q.w_g2;
   a.x = reinterpret_cast<const double2 *const *>(dtab);
-  a.params = P->ws->params.as<const int>();
+  a.params = reinterpret_cast<const int *>(dup);
   a.taps1 = P->v3_time1.as<const double2>();
   a.taps2 = P->v3_time2.as<const double2>();
   a.win1 = P->v3_freq1.as<const double2>();
@@ -691,8 +697,8 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   a.Fcap = P->v3_Fcap;
   a.Rcap = P->v3_Rcap;
   a.max_passes = 3 * 4096;  // the oracle's cap of the round-robin loop
-  a.count = P->ws->count.as<int64_t>();
-  a.status = P->ws->v3_status.as<int>();
+  a.count = reinterpret_cast<int64_t *>(ddn);
+  a.status = reinterpret_cast<int *>(ddn + cnt_bytes);
   const bool want_sparse = io.freq != nullptr || io.val != nullptr;
   a.o_freq = want_sparse ? reinterpret_cast<int32_t *const *>(dtab + tab_bytes) : nullptr;
   a.o_val = want_sparse ? reinterpret_cast<double2 *const *>(dtab + 2 * tab_bytes) : nullptr;
@@ -700,14 +706,6 @@ static int v3_prepare(Plan *P, BatchIO &io, hipStream_t st, V3Pending &pd)
   a.o_cap = io.cap;
   a.lds_bytes = P->v3_lds;
   P->last_S = S;
-  return 0;
-}
-
-// Counts and status flags back to the host (enqueued on st).
-static int v3_collect(const V3Pending &pd, int S, hipStream_t st)
-{
-  HIP_OK(hipMemcpyAsync(pd.h_counts, pd.a.count, (size_t)S * 8, hipMemcpyDeviceToHost, st));
-  HIP_OK(hipMemcpyAsync(pd.h_status, pd.a.status, (size_t)S * 4, hipMemcpyDeviceToHost, st));
   return 0;
 }
 
@@ -730,63 +728,77 @@ static int v3_finish(Plan *P, BatchIO &io, const V3Pending &pd)
 
 static int exec_batch_v3(Plan *P, BatchIO &io)
 {
-  hipStream_t st = P->stream;
-  V3Pending pd;
-  if (v3_prepare(P, io, st, pd) != 0) return -1;
-  if (P->timing && !P->ev_made) {
-    for (hipEvent_t &e : P->ev) HIP_OK(hipEventCreate(&e));
-    P->ev_made = true;
-  }
-  // through the multi-plan kernel with one plan: it reads the descriptor
-  // from device memory (a by-value kernel argument whose address the
-  // kernel takes is copied to scratch and reloaded inside its loops)
-  std::vector<int2> blocks(io.S);
-  for (int s = 0; s < io.S; s++) blocks[s] = make_int2(0, s);
-  if (!P->ws->multi.ensure(sizeof(k::V3Launch) + blocks.size() * sizeof(int2) + 256)) return -1;
-  auto *d_groups = P->ws->multi.as<k::V3Launch>();
-  auto *d_blocks = reinterpret_cast<int2 *>(
-      (reinterpret_cast<uintptr_t>(d_groups + 1) + 255) & ~uintptr_t(255));
-  if (P->timing) (void)hipEventRecord(P->ev[0], st);
-  if (k::v3_multi(&pd.a, 1, d_groups, d_blocks, blocks.data(), io.S, st) != 0) return -1;
-  if (P->timing) (void)hipEventRecord(P->ev[1], st);
-  if (v3_collect(pd, io.S, st) != 0) return -1;
-  HIP_OK(hipStreamSynchronize(st));
-  if (P->timing) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, P->ev[0], P->ev[1]) == hipSuccess) {
-      P->stage_ms[5] += ms;
-      P->stage_launches[5] += 1;
-    }
-  }
-  return v3_finish(P, io, pd);
+  Plan *plans[1] = {P};
+  return exec_multi_v3(plans, &io, 1, P->stream, P->ws, P);
 }
 
-int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws)
+static int ensure_pinned_ws(Workspace *ws, size_t bytes)
+{
+  if (bytes <= ws->pinned_bytes) return 0;
+  if (ws->pinned_view) {
+    set_error("host staging of the allocation-free exec too small");
+    return -1;
+  }
+  if (ws->pinned) (void)hipHostFree(ws->pinned);
+  ws->pinned = nullptr;
+  ws->pinned_bytes = 0;
+  HIP_OK(hipHostMalloc(&ws->pinned, bytes, hipHostMallocDefault));
+  ws->pinned_bytes = bytes;
+  return 0;
+}
+
+int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws,
+                  Plan *timer)
 {
   std::vector<V3Pending> pd(G);
   std::vector<k::V3Launch> groups(G);
   std::vector<int2> blocks;
+  size_t up = 0, dn = 0;
   for (int g = 0; g < G; g++) {
     if (plans[g]->version != SFFT_VERSION_3) {
       set_error("exec_multi_v3: not a v3 plan");
       return -1;
     }
-    if (v3_prepare(plans[g], ios[g], st, pd[g]) != 0) return -1;
-    groups[g] = pd[g].a;
     for (int s = 0; s < ios[g].S; s++) blocks.push_back(make_int2(g, s));
+    up += v3_up_bytes(ios[g].S);
+    dn += v3_dn_bytes(ios[g].S);
   }
   if (blocks.empty()) return 0;
-  if (!ws->multi.ensure(G * sizeof(k::V3Launch) + blocks.size() * sizeof(int2) + 256))
+  const size_t desc_bytes = al256((size_t)G * sizeof(k::V3Launch));
+  const size_t blk_bytes = al256(blocks.size() * sizeof(int2));
+  const size_t up_total = desc_bytes + blk_bytes + up;
+  if (!ws->multi.ensure(up_total + dn) || ensure_pinned_ws(ws, up_total + dn) != 0) return -1;
+  V3Stage sg;
+  sg.h = reinterpret_cast<char *>(ws->pinned);
+  sg.d = reinterpret_cast<char *>(ws->multi.p);
+  sg.up = desc_bytes + blk_bytes;
+  sg.dn = up_total;
+  for (int g = 0; g < G; g++) {
+    if (v3_prepare(plans[g], ios[g], pd[g], sg) != 0) return -1;
+    groups[g] = pd[g].a;
+  }
+  std::memcpy(sg.h, groups.data(), (size_t)G * sizeof(k::V3Launch));
+  std::memcpy(sg.h + desc_bytes, blocks.data(), blocks.size() * sizeof(int2));
+  HIP_OK(hipMemcpyAsync(sg.d, sg.h, up_total, hipMemcpyHostToDevice, st));
+  const bool timing = timer && timer->timing;
+  if (timing && !timer->ev_made) {
+    for (hipEvent_t &e : timer->ev) HIP_OK(hipEventCreate(&e));
+    timer->ev_made = true;
+  }
+  if (timing) (void)hipEventRecord(timer->ev[0], st);
+  if (k::v3_multi(groups.data(), G, reinterpret_cast<k::V3Launch *>(sg.d),
+                  reinterpret_cast<int2 *>(sg.d + desc_bytes), (int)blocks.size(), st) != 0)
     return -1;
-  auto *d_groups = ws->multi.as<k::V3Launch>();
-  auto *d_blocks = reinterpret_cast<int2 *>(
-      (reinterpret_cast<uintptr_t>(d_groups + G) + 255) & ~uintptr_t(255));
-  if (k::v3_multi(groups.data(), G, d_groups, d_blocks, blocks.data(), (int)blocks.size(),
-                  st) != 0)
-    return -1;
-  for (int g = 0; g < G; g++)
-    if (v3_collect(pd[g], ios[g].S, st) != 0) return -1;
+  if (timing) (void)hipEventRecord(timer->ev[1], st);
+  HIP_OK(hipMemcpyAsync(sg.h + up_total, sg.d + up_total, dn, hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
+  if (timing) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, timer->ev[0], timer->ev[1]) == hipSuccess) {
+      timer->stage_ms[5] += ms;
+      timer->stage_launches[5] += 1;
+    }
+  }
   int rc = 0;  // 1: some map overflowed (every result written)
   for (int g = 0; g < G; g++)
     if (v3_finish(plans[g], ios[g], pd[g]) != 0) rc = 1;

@@ -309,7 +309,8 @@ int exec_batch_wait(Plan *P, BatchIO &io, const V12Pending &pd);
 // v3 batches of G plans (different n, k) as one launch on stream st; the
 // descriptors live in ws (not any plan's).  Plans' own workspaces hold their
 // signals' state.  Returns 0, -1, or 1 (a map overflowed, results written).
-int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws);
+int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Workspace *ws,
+                  Plan *timer = nullptr);
 
 // ---------------------------------------------------------------- kernels
 namespace k {
@@ -421,10 +422,10 @@ struct V3Launch {
   int64_t o_cap;
   size_t lds_bytes;
 };
-// G groups in one launch; d_groups / d_blocks: device room for the
-// descriptors and the nblocks (group, signal) pairs.
-int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,
-             const int2 *h_blocks, int nblocks, hipStream_t st);
+// G groups in one launch; d_groups / d_blocks: the descriptors and the
+// nblocks (group, signal) pairs, already on the device (enqueued on st).
+int v3_multi(const V3Launch *h_groups, int G, const V3Launch *d_groups, const int2 *d_blocks,
+             int nblocks, hipStream_t st);
 
 // Batched power-of-two FFTs; transform t = (outer, inner) lives at
 // data + outer*so + inner*si.  In LDS for small forward sizes (tw: the

@@ -1220,20 +1220,13 @@ static void allow_v3_lds(const void *kernel, size_t bytes)
   if (bytes > 65536) raise_lds_limit(kernel);
 }
 
-int v3_multi(const V3Launch *h_groups, int G, V3Launch *d_groups, int2 *d_blocks,
-             const int2 *h_blocks, int nblocks, hipStream_t st)
+int v3_multi(const V3Launch *h_groups, int G, const V3Launch *d_groups, const int2 *d_blocks,
+             int nblocks, hipStream_t st)
 {
   size_t lds = 0;
   for (int g = 0; g < G; g++) {
     if (mansour_prepass(h_groups[g], st) != 0) return -1;
     lds = std::max(lds, h_groups[g].lds_bytes);
-  }
-  if (hipMemcpyAsync(d_groups, h_groups, (size_t)G * sizeof(V3Launch), hipMemcpyHostToDevice,
-                     st) != hipSuccess ||
-      hipMemcpyAsync(d_blocks, h_blocks, (size_t)nblocks * sizeof(int2), hipMemcpyHostToDevice,
-                     st) != hipSuccess) {
-    set_error("v3: descriptor upload failed");
-    return -1;
   }
   bool refine = false;
   int nsub = 1;
