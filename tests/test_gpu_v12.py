@@ -424,13 +424,14 @@ def test_many_plans_v12_mix_equals_alone(torch_cuda, precision):
 
 @pytest.mark.parametrize("graphs", [False, True])
 def test_parted_batch_equals_single(torch_cuda, graphs):
-    """A v2 FP32 batch of 96 signals of n=2^22, k=64 runs in parts (the fold
-    and bucket FFTs of one part beside the estimation of the previous one,
-    on two streams, eagerly or replayed from a HIP graph); every signal gets
-    the bits it gets alone (a single signal runs unparted)."""
+    """A v2 FP32 batch of 97 signals of n=2^22, k=64 runs in two uneven
+    parts (48 + 49: the fold and bucket FFTs of one part beside the
+    estimation of the previous one, on two streams, eagerly or replayed from
+    a HIP graph); every signal gets the bits it gets alone (a single signal
+    runs unparted)."""
     torch = torch_cuda
     from sfft_amd.signals import signals_device
-    n, k, S = 1 << 22, 64, 96
+    n, k, S = 1 << 22, 64, 97
     print(f"case parted batch n={n} k={k} S={S} graphs={graphs}", flush=True)
     x, _, _ = signals_device(n, k, list(range(700, 700 + S)), snr_db=40.0)
     batch = _gpu_plan(2, n, k, "fp32_refine")
