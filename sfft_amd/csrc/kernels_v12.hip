@@ -1007,9 +1007,9 @@ __global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0,
   }
 }
 
-// Residues per tile of k_estimate_rows (RI / 4 per wavefront): 32 for
+// Residues per tile of k_estimate_rows (RI / 4 per wavefront): 16 for
 // batches, 4 when a batch gives too few tiles to fill the chip.
-constexpr int kRI = 32, kRIsmall = 4;
+constexpr int kRI = 16, kRIsmall = 4;
 
 // FULL: L == LMAX, so the loop over loops has no branch and the compiler
 // issues all bucket loads of a frequency before the first wait.
