@@ -333,9 +333,10 @@ static int make_v3_tables(Plan *P)
   // fits
   const size_t gauss = (size_t)(2 * q.B_g1 + 2 * q.B_g2) * 16;
   P->v3_lds = std::max<size_t>(P->v3_lds,
-                               std::min<size_t>(gauss + (size_t)(q.w_g2 + 1) * 16, 160 * 1024));
-  if (P->v3_lds > 160 * 1024) {
-    set_error("v3 Gauss filters above 2560 buckets are not supported");
+                               std::min<size_t>(gauss + (size_t)(q.w_g2 + 1) * 16, kV3MaxDynLds));
+  if (P->v3_lds > kV3MaxDynLds) {
+    set_error("v3: the four Gauss bucket arrays (2 B_g1 + 2 B_g2 complex doubles) exceed the "
+              "LDS of a workgroup; k is too large for this plan");
     return -1;
   }
   const bool ok = upload(P->v3_time1, to_d2(f1.time, f1.time.size())) &&

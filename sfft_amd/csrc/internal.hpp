@@ -24,6 +24,9 @@ constexpr int kMaxLdsFft64 = 8192;
 constexpr int kMaxLoops = 32;
 // Mansour offsets of the v3 refine form (equations per bucket).
 constexpr int kV3RefineOffsets = 8;
+// Dynamic LDS of the v3 kernels: 160 KiB per workgroup less 4 KiB kept for
+// their static arrays (scan scratch, estimate ranks, loop flags)
+constexpr size_t kV3MaxDynLds = 160 * 1024 - 4096;
 
 void set_error(const std::string &msg);
 

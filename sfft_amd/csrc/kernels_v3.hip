@@ -394,27 +394,36 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
 }
 
 // ---------------------------------------------------------------- estimation
-// Candidates per bucket into cf/cv (cf = -1: none), then compaction in
-// bucket order into ef/ev.  Returns the found count (uniform).
-__device__ __forceinline__ int compact(const Ctx &c, int nb, int *sh)
+// An estimate's accepted buckets go to ef/ev in ascending bucket order (the
+// order the reference appends them in, hence the map's insertion order),
+// without a staging slot per bucket.
+
+// One step of kV3Block buckets (bucket = step start + threadIdx.x) whose
+// results are in registers: the accepted ones are written at base + their
+// rank among the step's accepted buckets.  Returns the step's count
+// (uniform).  Uses NW ints at sh; contains barriers.
+__device__ __forceinline__ int emit_step(const Ctx &c, int base, bool ok, int out, double2 val,
+                                         int *sh)
 {
-  const int T = blockDim.x;
-  const int C = (nb + T - 1) / T;
-  const int lo = threadIdx.x * C, hi = min(nb, lo + C);
-  int cnt = 0;
-  for (int i = lo; i < hi; i++) cnt += c.cf[i] >= 0;
-  int total;
-  int pos = block_exclusive_scan(cnt, sh, total);
-  for (int i = lo; i < hi; i++) {
-    if (c.cf[i] >= 0) {
-      c.ef[pos] = c.cf[i];
-      c.ev[pos] = c.cv[i];
-      pos++;
-    }
-  }
-  __threadfence_block();
+  constexpr int NW = kV3Block / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t bal = __ballot(ok);
+  if (lane == 0) sh[wid] = __popcll(bal);
   __syncthreads();
-  return total;
+  int off = base, tot = 0;
+#pragma unroll
+  for (int q = 0; q < NW; q++) {
+    const int t = sh[q];
+    off += q < wid ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  if (ok) {
+    const int pos = off + __popcll(bal & ((1ull << lane) - 1));
+    c.ef[pos] = out;
+    c.ev[pos] = val;
+  }
+  return tot;
 }
 
 // estimate_freq_gauss_loops2 (cf3 484-640) on (S0, S1), B buckets.
@@ -428,89 +437,148 @@ __device__ __forceinline__ int est_gauss(const Ctx &c, const double2 *S0, const 
   const double PI2_A_OFFSET_OVER_N = PI2 * a_ * c.gG / n;
   const double BUCKETS_OVER_N = (double)B / n;
   const int N_OVER_BUCKETS = n / B;
-  for (int bk = threadIdx.x; bk < B; bk += blockDim.x) {
+  int found = 0;
+  for (int s0 = 0; s0 < B; s0 += blockDim.x) {
+    const int bk = s0 + threadIdx.x;
     int out = -1;
     double2 val = make_double2(0, 0);
-    const double a0 = S0[bk].x, b0 = S0[bk].y, a1 = S1[bk].x, b1 = S1[bk].y;
-    const double zb = (a0 * a0 + b0 * b0) + (a1 * a1 + b1 * b1);
-    if (zb > 1e-8) {
-      const double c0 = a0 * a0 + b0 * b0, c1 = a1 * a1 + b1 * b1;
-      const double d0 = atan2(b0, a0), d1 = atan2(b1, a1);
-      const double eb = c1 / c0 - 1;
-      double err = eb * eb;
-      err /= n;
-      if (err < 1e-12 && c0 > 0.01) {
-        const double slope = (d1 - d0) * 1.0;
-        int freq = (int)rint(N_OVER_PI2 * slope) + n;
-        freq = (int)((unsigned)freq & n1);
-        const unsigned hashed = (unsigned)(long long)rint(freq * BUCKETS_OVER_N) & B1m;
-        if (hashed == (unsigned)bk) {
-          const double ph = d0 - PI2_A_OFFSET_OVER_N * freq;
-          const double mag = sqrt(c0);
-          double sn, cs;
-          sincos(ph, &sn, &cs);
-          const double mr = mag * cs, mi = mag * sn;
-          const unsigned dist =
-              (hashed * (unsigned)N_OVER_BUCKETS - (unsigned)freq + (unsigned)n) & n1;
-          const double2 F = win_at(win, half, dist, n);
-          const double den = F.x * F.x + F.y * F.y;
-          val = make_double2((mr * F.x + mi * F.y) / den, (mi * F.x - mr * F.y) / den);
-          out = (int)(((unsigned)(timesmod(freq, a_, n) - b_ + n)) & n1);
+    if (bk < B) {
+      const double a0 = S0[bk].x, b0 = S0[bk].y, a1 = S1[bk].x, b1 = S1[bk].y;
+      const double zb = (a0 * a0 + b0 * b0) + (a1 * a1 + b1 * b1);
+      if (zb > 1e-8) {
+        const double c0 = a0 * a0 + b0 * b0, c1 = a1 * a1 + b1 * b1;
+        const double d0 = atan2(b0, a0), d1 = atan2(b1, a1);
+        const double eb = c1 / c0 - 1;
+        double err = eb * eb;
+        err /= n;
+        if (err < 1e-12 && c0 > 0.01) {
+          const double slope = (d1 - d0) * 1.0;
+          int freq = (int)rint(N_OVER_PI2 * slope) + n;
+          freq = (int)((unsigned)freq & n1);
+          const unsigned hashed = (unsigned)(long long)rint(freq * BUCKETS_OVER_N) & B1m;
+          if (hashed == (unsigned)bk) {
+            const double ph = d0 - PI2_A_OFFSET_OVER_N * freq;
+            const double mag = sqrt(c0);
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const double mr = mag * cs, mi = mag * sn;
+            const unsigned dist =
+                (hashed * (unsigned)N_OVER_BUCKETS - (unsigned)freq + (unsigned)n) & n1;
+            const double2 F = win_at(win, half, dist, n);
+            const double den = F.x * F.x + F.y * F.y;
+            val = make_double2((mr * F.x + mi * F.y) / den, (mi * F.x - mr * F.y) / den);
+            out = (int)(((unsigned)(timesmod(freq, a_, n) - b_ + n)) & n1);
+          }
         }
       }
     }
-    c.cf[bk] = out;
-    c.cv[bk] = val;
+    found += emit_step(c, found, out >= 0, out, val, sh);
   }
   __threadfence_block();
   __syncthreads();
-  return compact(c, B, sh);
+  return found;
 }
 
-// estimate_freq_mansour_loops2 (cf3 642-776)
-__device__ __forceinline__ int est_mansour(const Ctx &c, int *sh)
+// estimate_freq_mansour_loops2 (cf3 642-776) for bucket bk: the acceptance
+// test, and with Full the frequency and the value.  The test needs only the
+// bucket powers, so a sweep can evaluate it alone and the full estimate
+// later for the accepted buckets (the same expressions, so the same bits).
+struct MansourConst {
+  double N_OVER_PI2, PI2_OVER_N, NORM, NORM2, err_max;
+  unsigned FREQ_MASK;
+};
+
+template <bool Full>
+__device__ __forceinline__ bool mansour_bucket(const Ctx &c, const MansourConst &m, int bk,
+                                               int &out, double2 &val)
 {
+  const double2 s0 = c.MS[2 * bk], s1 = c.MS[2 * bk + 1];
+  const double a0 = s0.x, b0 = s0.y, a1 = s1.x, b1 = s1.y;
+  const double zb = (a0 * a0 + b0 * b0) * m.NORM2 + (a1 * a1 + b1 * b1) * m.NORM2;
+  if (!(zb > 1e-8)) return false;
+  const double c0 = (a0 * a0 + b0 * b0) * m.NORM2;
+  const double c1 = (a1 * a1 + b1 * b1) * m.NORM2;
+  double median_abs = c0;
+  const double inv = 1. / median_abs;
+  const double bb = (c1 * inv) - 1;
+  const double err = bb * bb;
+  if (!(err < m.err_max && median_abs > 0.01)) return false;
+  if (Full) {
+    const double d0 = atan2(b0 * m.NORM, a0 * m.NORM), d1 = atan2(b1 * m.NORM, a1 * m.NORM);
+    const double slope = d1 - d0;
+    const int f1 = (int)rint(slope * m.N_OVER_PI2);
+    const unsigned f2 = (unsigned)f1 & m.FREQ_MASK;
+    const int f3 = (int)(f2 | (unsigned)bk);
+    const int fo = wmul(f3, c.g0);
+    const double ph = d0 - m.PI2_OVER_N * fo;
+    median_abs = sqrt(median_abs);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    out = f3;
+    val = make_double2(median_abs * cs, median_abs * sn);
+  }
+  return true;
+}
+
+// The W Mansour buckets swept in segments of 64 steps of kV3Block buckets
+// (bucket = segment start + step kV3Block + threadIdx.x).  The first sweep
+// evaluates only the acceptance test: each thread keeps its accepted steps
+// as bits, each wave writes the count of a step to rank[step NW + wave]; a
+// block scan over rank (which is bucket order) gives every (step, wave) its
+// first slot; the second sweep estimates the accepted buckets and writes
+// each at that slot plus its accepted lanes below.  rank holds kV3Block
+// ints of LDS.
+__device__ __forceinline__ int est_mansour(const Ctx &c, int *rank, int *sh)
+{
+  constexpr int T = kV3Block, NW = T / 64, SEG = 64;
   const int n = c.n, W = c.W;
   const double PI2 = 2 * M_PI;
-  const double N_OVER_PI2 = n / PI2;
-  const double PI2_OVER_N = PI2 / n;
-  const unsigned FREQ_MASK = ((unsigned)n - 1) & ~((unsigned)W - 1);
-  const double NORM = 1. / W;
-  const double NORM2 = NORM * NORM;
-  for (int bk = threadIdx.x; bk < W; bk += blockDim.x) {
-    int out = -1;
-    double2 val = make_double2(0, 0);
-    const double2 s0 = c.MS[2 * bk], s1 = c.MS[2 * bk + 1];
-    const double a0 = s0.x, b0 = s0.y, a1 = s1.x, b1 = s1.y;
-    const double zb = (a0 * a0 + b0 * b0) * NORM2 + (a1 * a1 + b1 * b1) * NORM2;
-    if (zb > 1e-8) {
-      const double c0 = (a0 * a0 + b0 * b0) * NORM2;
-      const double c1 = (a1 * a1 + b1 * b1) * NORM2;
-      const double d0 = atan2(b0 * NORM, a0 * NORM), d1 = atan2(b1 * NORM, a1 * NORM);
-      double median_abs = c0;
-      const double inv = 1. / median_abs;
-      const double bb = (c1 * inv) - 1;
-      const double err = bb * bb;
-      if (err < n * 1e-10 && median_abs > 0.01) {
-        const double slope = d1 - d0;
-        const int f1 = (int)rint(slope * N_OVER_PI2);
-        const unsigned f2 = (unsigned)f1 & FREQ_MASK;
-        const int f3 = (int)(f2 | (unsigned)bk);
-        const int fo = wmul(f3, c.g0);
-        const double ph = d0 - PI2_OVER_N * fo;
-        median_abs = sqrt(median_abs);
-        double sn, cs;
-        sincos(ph, &sn, &cs);
-        out = f3;
-        val = make_double2(median_abs * cs, median_abs * sn);
+  MansourConst m;
+  m.N_OVER_PI2 = n / PI2;
+  m.PI2_OVER_N = PI2 / n;
+  m.FREQ_MASK = ((unsigned)n - 1) & ~((unsigned)W - 1);
+  m.NORM = 1. / W;
+  m.NORM2 = m.NORM * m.NORM;
+  m.err_max = n * 1e-10;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int found = 0;
+  for (int s0 = 0; s0 < W; s0 += SEG * T) {
+    const int steps = min(SEG, (W - s0 + T - 1) / T);
+    uint64_t mine = 0;
+    for (int st = 0; st < steps; st++) {
+      const int bk = s0 + st * T + threadIdx.x;
+      int o;
+      double2 v;
+      const bool ok = bk < W && mansour_bucket<false>(c, m, bk, o, v);
+      mine |= (uint64_t)ok << st;
+      const uint64_t bal = __ballot(ok);
+      if (lane == 0) rank[st * NW + wid] = __popcll(bal);
+    }
+    __syncthreads();
+    const int nr = steps * NW;
+    int total;
+    const int first = block_exclusive_scan(threadIdx.x < nr ? rank[threadIdx.x] : 0, sh, total);
+    if (threadIdx.x < nr) rank[threadIdx.x] = found + first;
+    __syncthreads();
+    for (int st = 0; st < steps; st++) {
+      const bool ok = (mine >> st) & 1;
+      const uint64_t bal = __ballot(ok);
+      if (ok) {
+        const int bk = s0 + st * T + threadIdx.x;
+        int o = -1;
+        double2 v = make_double2(0, 0);
+        (void)mansour_bucket<true>(c, m, bk, o, v);
+        const int pos = rank[st * NW + wid] + __popcll(bal & ((1ull << lane) - 1));
+        c.ef[pos] = o;
+        c.ev[pos] = v;
       }
     }
-    c.cf[bk] = out;
-    c.cv[bk] = val;
+    found += total;
+    __syncthreads();  // rank is rewritten by the next segment
   }
   __threadfence_block();
   __syncthreads();
-  return compact(c, W, sh);
+  return found;
 }
 
 // Adds the F found (ef, ev) to the map (new keys appended in found order).
@@ -781,6 +849,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
 {
   extern __shared__ double2 lds[];
   __shared__ int sh[16];
+  __shared__ int s_rank[kV3Block];
   __shared__ int s_size, s_flag, s_cnt[3];
   Ctx c;
   c.a = &a;
@@ -820,7 +889,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   __syncthreads();
 
   // ---- Mansour filter (cf3 96-127): buckets from k_v3_pre
-  int F = est_mansour(c, sh);
+  int F = est_mansour(c, s_rank, sh);
   if (!map_add_found(c, F, sh, &s_size)) {
     if (threadIdx.x == 0) s_flag |= 2;
   }
@@ -874,7 +943,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   int pass = 0;
   for (; pass < a.max_passes; pass++) {
     const int sel = pass % 3;
-    if (sel == 0) F = est_mansour(c, sh);
+    if (sel == 0) F = est_mansour(c, s_rank, sh);
     else if (sel == 1) F = est_gauss(c, c.GS0, c.GS1, a.B1, 1, 0, a.win1, a.win1_half, sh);
     else F = est_gauss(c, c.GP0, c.GP1, a.B2, c.aa, c.b, a.win2, a.win2_half, sh);
     if (!map_add_found(c, F, sh, &s_size)) {
@@ -896,9 +965,11 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
         const double2 z = (j & 1) ? c.GP1[j >> 1] : c.GP0[j >> 1];
         c2 += z.x * z.x + z.y * z.y > 1e-6;
       }
+      // W is a power of two: z / W and z * (1 / W) round the same value
+      const double invW = 1.0 / W;
       for (int j = threadIdx.x; j < W; j += blockDim.x) {
         const double2 z = c.MS[2 * j];
-        const double re = z.x / W, im = z.y / W;
+        const double re = z.x * invW, im = z.y * invW;
         cm += re * re + im * im > 1e-6;
       }
       atomicAdd(&s_cnt[0], cm);

@@ -21,6 +21,7 @@ CASES = [
     (1 << 24, 200),     # one signal of the n=2^24 v3 workload
     (1 << 24, 1000),    # the batched v3 workload's largest k
     (1 << 20, 2000),    # W_Man = 16384: Mansour FFTs before the kernel
+    (1 << 20, 3000),    # B_g1 = 2048: Gauss-2 phase chain in HBM, largest LDS
 ]
 
 
