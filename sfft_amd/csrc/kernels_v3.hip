@@ -30,6 +30,8 @@ static_assert((kV3Block & (kV3Block - 1)) == 0, "owner tests take residues by ma
 constexpr int kSerialApply = 16;
 // Bucket indices an owner thread loads ahead in the subtraction scans.
 constexpr int kApplyAhead = 32;
+// Steps of a sweep over the Mansour buckets whose loads are issued together.
+constexpr int kSweepAhead = 4;
 // Pairs per LDS chunk of the Gauss subtractions (fewer fitting than
 // kMinStageChunk: indices and values read from memory instead); a chunk is
 // a multiple of 32, the owners' scan width.
@@ -202,10 +204,10 @@ __device__ void gauss_apply_owned(double2 *__restrict__ S0, double2 *__restrict_
 // As gauss_apply_owned, a chunk of pairs at a time through LDS: the block
 // copies the chunk's bucket indices and removal values next to the bucket
 // arrays (one memory latency per chunk), then the owners apply them in pair
-// order from LDS.  With B <= blockDim each thread owns one bucket and keeps
-// it in registers over the whole pass (written back at the end); it marks
-// the pairs that touch its bucket among each 32 in a bit mask and applies
-// just those, so the threads of a wave work on different pairs at once.
+// order from LDS.  Thread t owns buckets t, t + blockDim, ... and takes
+// them one at a time in registers: it marks the pairs that touch the bucket
+// among each 32 in a bit mask and applies just those, so the threads of a
+// wave work on different pairs at once.
 // The subtractions a bucket sees and their order are unchanged.  Called by
 // every thread of the block (barriers inside).
 __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict__ S1, int B,
@@ -216,13 +218,7 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
 {
   constexpr int T = kV3Block;
   const int t = threadIdx.x;
-  const int M = min(B, T);
-  const bool one = B <= T;  // one bucket per owner thread (B a power of two)
-  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
-  if (one && t < B) {
-    a0 = S0[t];
-    a1 = S1[t];
-  }
+  const int bm = B - 1;  // B a power of two
   for (int base = 0; base < cnt; base += chunk) {
     const int m = min(chunk, cnt - base);
     __syncthreads();  // the previous chunk is applied
@@ -232,64 +228,47 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
       sv[j] = rem[(int64_t)(base + u) * 14 + roff + q];
     }
     __syncthreads();
-    if (one) {
-      if (t < B) {
-        const int bm = B - 1;
-        for (int g0 = 0; g0 < m; g0 += 32) {
-          // this thread's pairs among the next 32, as a bit mask ... (the
-          // chunk is a multiple of 32, so all 32 reads stay in sh_idx;
-          // entries past m are stale and masked off)
-          const int gm = min(32, m - g0);
-          unsigned own = 0;
+    // thread t owns the buckets tb = t + i T, one at a time in registers
+    for (int tb = t; tb < B; tb += T) {
+      double2 a0 = S0[tb], a1 = S1[tb];
+      for (int g0 = 0; g0 < m; g0 += 32) {
+        // this bucket's pairs among the next 32, as a bit mask ... (the
+        // chunk is a multiple of 32, so all 32 reads stay in sh_idx;
+        // entries past m are stale and masked off)
+        const int gm = min(32, m - g0);
+        unsigned own = 0;
 #pragma unroll
-          for (int uu = 0; uu < 32; uu++) {
-            // t is one of h, h+1, h-1 mod B (exact for every power of two)
-            const int h = sh_idx[g0 + uu];
-            own |= (unsigned)((unsigned)((t - h + 1) & bm) <= 2u) << uu;
+        for (int uu = 0; uu < 32; uu++) {
+          // tb is one of h, h+1, h-1 mod B (exact for every power of two)
+          const int h = sh_idx[g0 + uu];
+          own |= (unsigned)((unsigned)((tb - h + 1) & bm) <= 2u) << uu;
+        }
+        if (gm < 32) own &= (1u << gm) - 1u;
+        // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482);
+        // the pair's values are loaded with its index, not after it
+        while (own) {
+          const int u = g0 + __ffs(own) - 1;
+          own &= own - 1;
+          const double2 *r = sv + 6 * u;
+          const int h = sh_idx[u];
+          const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+          if (h == tb) {
+            a0 = csub(a0, r0);
+            a1 = csub(a1, r3);
           }
-          if (gm < 32) own &= (1u << gm) - 1u;
-          // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482);
-          // the pair's values are loaded with its index, not after it
-          while (own) {
-            const int u = g0 + __ffs(own) - 1;
-            own &= own - 1;
-            const double2 *r = sv + 6 * u;
-            const int h = sh_idx[u];
-            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-            if (h == t) {
-              a0 = csub(a0, r0);
-              a1 = csub(a1, r3);
-            }
-            if (((h + 1) & bm) == t) {
-              a0 = csub(a0, r1);
-              a1 = csub(a1, r4);
-            }
-            if (((h + bm) & bm) == t) {
-              a0 = csub(a0, r2);
-              a1 = csub(a1, r5);
-            }
+          if (((h + 1) & bm) == tb) {
+            a0 = csub(a0, r1);
+            a1 = csub(a1, r4);
+          }
+          if (((h + bm) & bm) == tb) {
+            a0 = csub(a0, r2);
+            a1 = csub(a1, r5);
           }
         }
       }
-      continue;
+      S0[tb] = a0;
+      S1[tb] = a1;
     }
-    if (t < B) {
-      for (int u = 0; u < m; u++) {
-        const int h = sh_idx[u];
-        if ((unsigned)((h - t + 1) & (M - 1)) > 2u) continue;
-        const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-          if (bk[q] % T != t) continue;
-          S0[bk[q]] = csub(S0[bk[q]], sv[6 * u + q]);
-          S1[bk[q]] = csub(S1[bk[q]], sv[6 * u + 3 + q]);
-        }
-      }
-    }
-  }
-  if (one && t < B) {
-    S0[t] = a0;
-    S1[t] = a1;
   }
   __syncthreads();
 }
@@ -315,6 +294,48 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
                                          int cnt, int which)
 {
   const V3Launch &a = *c.a;
+  extern __shared__ double2 lds[];
+  // LDS past the four Gauss bucket arrays (staging for the applications)
+  const size_t used = (2 * (size_t)c.B1 + 2 * (size_t)c.B2) * sizeof(double2);
+  const size_t room = a.lds_bytes > used ? a.lds_bytes - used : 0;
+  double2 *const sv = lds + 2 * c.B1 + 2 * c.B2;
+  if (cnt <= kSerialApply && !a.serial_all &&
+      room >= (size_t)kSerialApply * (14 * sizeof(double2) + 3 * sizeof(int))) {
+    // Few pairs: the removal values stay in LDS; one thread per Gauss
+    // bucket array applies them in pair order, and each Mansour bucket
+    // (in HBM) is updated by its owner thread h mod kV3Block in pair order
+    // -- every bucket sees the same subtractions in the same order.
+    int *si = reinterpret_cast<int *>(sv + 14 * kSerialApply);
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const int key = keys[i];
+      const double2 v = vals[i];
+      double2 *r = sv + i * 14;
+      int *ri = si + i * 3;
+      if (which & 1) gauss_removal(c, key, v, c.B1, a.win1, a.win1_half, c.gG, 1, ri[0], r);
+      if (which & 2)
+        gauss_removal(c, (timesmod(key, c.ai, c.n) + c.shift) % c.n, v, c.B2, a.win2,
+                      a.win2_half, wmul(c.aa, c.gG), 1, ri[1], r + 6);
+      if (which & 4) mansour_removal(c, key, v, ri[2], r + 12);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (which & 1)) {
+      for (int i = 0; i < cnt; i++) gauss_apply_pair(c.GS0, c.GS1, c.B1, si[i * 3], sv + i * 14);
+    } else if (threadIdx.x == 64 && (which & 2)) {
+      for (int i = 0; i < cnt; i++)
+        gauss_apply_pair(c.GP0, c.GP1, c.B2, si[i * 3 + 1], sv + i * 14 + 6);
+    }
+    if (which & 4) {
+      for (int i = 0; i < cnt; i++) {
+        const int h = si[i * 3 + 2];
+        if ((h & (kV3Block - 1)) != (int)threadIdx.x) continue;
+        c.MS[2 * h] = csub(c.MS[2 * h], sv[i * 14 + 12]);
+        c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], sv[i * 14 + 13]);
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    return;
+  }
   for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
     const int key = keys[i];
     const double2 v = vals[i];
@@ -351,13 +372,9 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
     return;
   }
   {
-    // LDS past the four Gauss bucket arrays: 6 double2 + 1 int per pair
-    extern __shared__ double2 lds[];
-    const size_t used = (2 * (size_t)c.B1 + 2 * (size_t)c.B2) * sizeof(double2);
-    const size_t room = a.lds_bytes > used ? a.lds_bytes - used : 0;
+    // staged in LDS: 6 double2 + 1 int per pair
     const int chunk =
         (int)min((size_t)kStageChunk, room / (6 * sizeof(double2) + sizeof(int))) / 32 * 32;
-    double2 *sv = lds + 2 * c.B1 + 2 * c.B2;
     int *sh_idx = reinterpret_cast<int *>(sv + 6 * (size_t)chunk);
     if (chunk >= kMinStageChunk) {
       if (which & 1) gauss_apply_staged(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt, sv, sh_idx, chunk);
@@ -490,9 +507,8 @@ struct MansourConst {
 
 template <bool Full>
 __device__ __forceinline__ bool mansour_bucket(const Ctx &c, const MansourConst &m, int bk,
-                                               int &out, double2 &val)
+                                               double2 s0, double2 s1, int &out, double2 &val)
 {
-  const double2 s0 = c.MS[2 * bk], s1 = c.MS[2 * bk + 1];
   const double a0 = s0.x, b0 = s0.y, a1 = s1.x, b1 = s1.y;
   const double zb = (a0 * a0 + b0 * b0) * m.NORM2 + (a1 * a1 + b1 * b1) * m.NORM2;
   if (!(zb > 1e-8)) return false;
@@ -522,8 +538,10 @@ __device__ __forceinline__ bool mansour_bucket(const Ctx &c, const MansourConst 
 
 // The W Mansour buckets swept in segments of 64 steps of kV3Block buckets
 // (bucket = segment start + step kV3Block + threadIdx.x).  The first sweep
-// evaluates only the acceptance test: each thread keeps its accepted steps
-// as bits, each wave writes the count of a step to rank[step NW + wave]; a
+// evaluates only the acceptance test, on kSweepAhead steps' buckets loaded
+// together (one memory latency for them): each thread keeps its accepted
+// steps as bits, each wave writes the count of a step to rank[step NW +
+// wave]; a
 // block scan over rank (which is bucket order) gives every (step, wave) its
 // first slot; the second sweep estimates the accepted buckets and writes
 // each at that slot plus its accepted lanes below.  rank holds kV3Block
@@ -545,14 +563,27 @@ __device__ __forceinline__ int est_mansour(const Ctx &c, int *rank, int *sh)
   for (int s0 = 0; s0 < W; s0 += SEG * T) {
     const int steps = min(SEG, (W - s0 + T - 1) / T);
     uint64_t mine = 0;
-    for (int st = 0; st < steps; st++) {
-      const int bk = s0 + st * T + threadIdx.x;
-      int o;
-      double2 v;
-      const bool ok = bk < W && mansour_bucket<false>(c, m, bk, o, v);
-      mine |= (uint64_t)ok << st;
-      const uint64_t bal = __ballot(ok);
-      if (lane == 0) rank[st * NW + wid] = __popcll(bal);
+    for (int st0 = 0; st0 < steps; st0 += kSweepAhead) {
+      double2 z0[kSweepAhead], z1[kSweepAhead];
+#pragma unroll
+      for (int u = 0; u < kSweepAhead; u++) {
+        // clamped: past the end the loads repeat bucket W - 1 unused
+        const int bk = min(s0 + (st0 + u) * T + (int)threadIdx.x, W - 1);
+        z0[u] = c.MS[2 * bk];
+        z1[u] = c.MS[2 * bk + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < kSweepAhead; u++) {
+        const int st = st0 + u;
+        if (st >= steps) break;
+        const int bk = s0 + st * T + threadIdx.x;
+        int o;
+        double2 v;
+        const bool ok = bk < W && mansour_bucket<false>(c, m, bk, z0[u], z1[u], o, v);
+        mine |= (uint64_t)ok << st;
+        const uint64_t bal = __ballot(ok);
+        if (lane == 0) rank[st * NW + wid] = __popcll(bal);
+      }
     }
     __syncthreads();
     const int nr = steps * NW;
@@ -567,7 +598,7 @@ __device__ __forceinline__ int est_mansour(const Ctx &c, int *rank, int *sh)
         const int bk = s0 + st * T + threadIdx.x;
         int o = -1;
         double2 v = make_double2(0, 0);
-        (void)mansour_bucket<true>(c, m, bk, o, v);
+        (void)mansour_bucket<true>(c, m, bk, c.MS[2 * bk], c.MS[2 * bk + 1], o, v);
         const int pos = rank[st * NW + wid] + __popcll(bal & ((1ull << lane) - 1));
         c.ef[pos] = o;
         c.ev[pos] = v;
@@ -967,10 +998,15 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
       }
       // W is a power of two: z / W and z * (1 / W) round the same value
       const double invW = 1.0 / W;
-      for (int j = threadIdx.x; j < W; j += blockDim.x) {
-        const double2 z = c.MS[2 * j];
-        const double re = z.x * invW, im = z.y * invW;
-        cm += re * re + im * im > 1e-6;
+      for (int j0 = threadIdx.x; j0 < W; j0 += kSweepAhead * kV3Block) {
+        double2 z[kSweepAhead];  // loaded together, clamped past the end
+#pragma unroll
+        for (int u = 0; u < kSweepAhead; u++) z[u] = c.MS[2 * min(j0 + u * kV3Block, W - 1)];
+#pragma unroll
+        for (int u = 0; u < kSweepAhead; u++) {
+          const double re = z[u].x * invW, im = z[u].y * invW;
+          cm += j0 + u * kV3Block < W && re * re + im * im > 1e-6;
+        }
       }
       atomicAdd(&s_cnt[0], cm);
       atomicAdd(&s_cnt[1], c1);
