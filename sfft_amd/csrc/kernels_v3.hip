@@ -33,9 +33,8 @@ constexpr int kApplyAhead = 32;
 // Steps of a sweep over the Mansour buckets whose loads are issued together.
 constexpr int kSweepAhead = 4;
 // Pairs per LDS chunk of the Gauss subtractions (fewer fitting than
-// kMinStageChunk: indices and values read from memory instead); a chunk is
-// a multiple of 32, the owners' scan width.
-constexpr int kStageChunk = 256, kMinStageChunk = 32;
+// kMinStageChunk: indices and values read from memory instead).
+constexpr int kStageChunk = 1024, kMinStageChunk = 32;
 
 __device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
 
@@ -201,24 +200,34 @@ __device__ void gauss_apply_owned(double2 *__restrict__ S0, double2 *__restrict_
   }
 }
 
+// LDS bytes the staged application needs per pair of a chunk: 6 removal
+// values, the bucket index, three sort entries.
+constexpr size_t kStagePairBytes = 6 * sizeof(double2) + sizeof(int) + 3 * sizeof(int);
+
 // As gauss_apply_owned, a chunk of pairs at a time through LDS: the block
 // copies the chunk's bucket indices and removal values next to the bucket
-// arrays (one memory latency per chunk), then the owners apply them in pair
-// order from LDS.  Thread t owns buckets t, t + blockDim, ... and takes
-// them one at a time in registers: it marks the pairs that touch the bucket
-// among each 32 in a bit mask and applies just those, so the threads of a
-// wave work on different pairs at once.
-// The subtractions a bucket sees and their order are unchanged.  Called by
+// arrays (one memory latency per chunk) and sorts the chunk's 3m entries
+// e = 3 u + q (pair u, q = 0, 1, 2 for its buckets h, h+1, h-1) by bucket:
+// counts, an exclusive scan, atomic placement, then each bucket's short
+// list re-sorted by e.  The owner of a bucket applies its entries in e
+// order, which is pair order and h, h+1, h-1 within a pair (cf3 465-482):
+// the subtractions a bucket sees and their order are unchanged.  stage
+// holds chunk * kStagePairBytes + (B + 16) * 4 bytes of LDS.  Called by
 // every thread of the block (barriers inside).
 __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict__ S1, int B,
                                    const int *__restrict__ remi, int slot,
                                    const double2 *__restrict__ rem, int roff, int cnt,
-                                   double2 *__restrict__ sv, int *__restrict__ sh_idx,
-                                   int chunk)
+                                   double2 *__restrict__ stage, int chunk)
 {
   constexpr int T = kV3Block;
   const int t = threadIdx.x;
   const int bm = B - 1;  // B a power of two
+  double2 *__restrict__ sv = stage;
+  int *__restrict__ sh_idx = reinterpret_cast<int *>(sv + 6 * (size_t)chunk);
+  int *__restrict__ list = sh_idx + chunk;
+  int *__restrict__ pos = list + 3 * chunk;  // B bucket counts, then ends
+  int *__restrict__ sh = pos + B;            // scan scratch
+  const int C = (B + T - 1) / T;             // buckets per thread in the scan
   for (int base = 0; base < cnt; base += chunk) {
     const int m = min(chunk, cnt - base);
     __syncthreads();  // the previous chunk is applied
@@ -227,44 +236,52 @@ __device__ void gauss_apply_staged(double2 *__restrict__ S0, double2 *__restrict
       const int u = j / 6, q = j - 6 * u;
       sv[j] = rem[(int64_t)(base + u) * 14 + roff + q];
     }
+    for (int b = t; b < B; b += T) pos[b] = 0;
     __syncthreads();
-    // thread t owns the buckets tb = t + i T, one at a time in registers
+    for (int e = t; e < 3 * m; e += T) {
+      const int u = e / 3, q = e - 3 * u;
+      const int h = sh_idx[u];
+      const int b = q == 0 ? h : (q == 1 ? (h + 1) & bm : (h + bm) & bm);
+      atomicAdd(&pos[b], 1);
+    }
+    __syncthreads();
+    {
+      const int lo = t * C, hi = min(B, lo + C);
+      int sum = 0;
+      for (int b = lo; b < hi; b++) sum += pos[b];
+      int total;
+      int run = block_exclusive_scan(sum, sh, total);
+      for (int b = lo; b < hi; b++) {
+        const int v = pos[b];
+        pos[b] = run;  // start of bucket b's entries
+        run += v;
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < 3 * m; e += T) {
+      const int u = e / 3, q = e - 3 * u;
+      const int h = sh_idx[u];
+      const int b = q == 0 ? h : (q == 1 ? (h + 1) & bm : (h + bm) & bm);
+      list[atomicAdd(&pos[b], 1)] = e;  // pos[b] ends at the end of bucket b
+    }
+    __syncthreads();
     for (int tb = t; tb < B; tb += T) {
+      const int lo = tb ? pos[tb - 1] : 0, hi = pos[tb];
+      for (int i = lo + 1; i < hi; i++) {  // insertion sort by e (short lists)
+        const int v = list[i];
+        int j = i - 1;
+        while (j >= lo && list[j] > v) {
+          list[j + 1] = list[j];
+          j--;
+        }
+        list[j + 1] = v;
+      }
       double2 a0 = S0[tb], a1 = S1[tb];
-      for (int g0 = 0; g0 < m; g0 += 32) {
-        // this bucket's pairs among the next 32, as a bit mask ... (the
-        // chunk is a multiple of 32, so all 32 reads stay in sh_idx;
-        // entries past m are stale and masked off)
-        const int gm = min(32, m - g0);
-        unsigned own = 0;
-#pragma unroll
-        for (int uu = 0; uu < 32; uu++) {
-          // tb is one of h, h+1, h-1 mod B (exact for every power of two)
-          const int h = sh_idx[g0 + uu];
-          own |= (unsigned)((unsigned)((tb - h + 1) & bm) <= 2u) << uu;
-        }
-        if (gm < 32) own &= (1u << gm) - 1u;
-        // ... applied in pair order: buckets h, h+1, h-1 (cf3 465-482);
-        // the pair's values are loaded with its index, not after it
-        while (own) {
-          const int u = g0 + __ffs(own) - 1;
-          own &= own - 1;
-          const double2 *r = sv + 6 * u;
-          const int h = sh_idx[u];
-          const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-          if (h == tb) {
-            a0 = csub(a0, r0);
-            a1 = csub(a1, r3);
-          }
-          if (((h + 1) & bm) == tb) {
-            a0 = csub(a0, r1);
-            a1 = csub(a1, r4);
-          }
-          if (((h + bm) & bm) == tb) {
-            a0 = csub(a0, r2);
-            a1 = csub(a1, r5);
-          }
-        }
+      for (int i = lo; i < hi; i++) {
+        const int e = list[i];
+        const int u = e / 3, q = e - 3 * u;
+        a0 = csub(a0, sv[6 * u + q]);
+        a1 = csub(a1, sv[6 * u + 3 + q]);
       }
       S0[tb] = a0;
       S1[tb] = a1;
@@ -372,13 +389,13 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
     return;
   }
   {
-    // staged in LDS: 6 double2 + 1 int per pair
+    // staged in LDS (kStagePairBytes per pair plus the bucket counts)
+    const size_t fixed = (size_t)(max(c.B1, c.B2) + 16) * sizeof(int);
     const int chunk =
-        (int)min((size_t)kStageChunk, room / (6 * sizeof(double2) + sizeof(int))) / 32 * 32;
-    int *sh_idx = reinterpret_cast<int *>(sv + 6 * (size_t)chunk);
+        room > fixed ? (int)min((size_t)kStageChunk, (room - fixed) / kStagePairBytes) : 0;
     if (chunk >= kMinStageChunk) {
-      if (which & 1) gauss_apply_staged(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt, sv, sh_idx, chunk);
-      if (which & 2) gauss_apply_staged(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt, sv, sh_idx, chunk);
+      if (which & 1) gauss_apply_staged(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt, sv, chunk);
+      if (which & 2) gauss_apply_staged(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt, sv, chunk);
     } else {
       if (which & 1) gauss_apply_owned(c.GS0, c.GS1, c.B1, c.remi, 0, c.rem, 0, cnt);
       if (which & 2) gauss_apply_owned(c.GP0, c.GP1, c.B2, c.remi, 1, c.rem, 6, cnt);
