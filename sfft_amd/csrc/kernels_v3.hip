@@ -1049,13 +1049,55 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   write_result(a, s, c.keys, c.vals, s_size, s_flag, lds);
 }
 
+// Ascending bitonic sort of P (a power of two) distinct u64 keys in LDS by
+// the whole block.  Each wave takes a contiguous share of the P/2
+// compare-exchange pairs of a stage, q = wave ppw + r; a pair (i, i + j) of
+// a stage with j <= ppw then lies in the wave's own span of 2 ppw keys, so
+// consecutive such stages need only the wave's own ordering (its LDS
+// operations complete in order) and block barriers surround only the
+// stages with a wider j.  Contains barriers.
+__device__ void block_bitonic_u64(uint64_t *v, int P)
+{
+  constexpr int NW = kV3Block / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int half = P >> 1;
+  const int ppw = half >= NW ? half / NW : half;  // pairs per wave in use
+  const int nwu = ppw ? half / ppw : 0;           // waves in use
+  bool prev_local = false;
+  for (int k2 = 2; k2 <= P; k2 <<= 1) {
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+      const bool local = j <= ppw;
+      if (local && prev_local) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      } else {
+        __syncthreads();
+      }
+      prev_local = local;
+      if (wid >= nwu) continue;
+      const int lj = ilog2v(j);
+      for (int r = lane; r < ppw; r += 64) {
+        const int q = wid * ppw + r;
+        const int i = ((q >> lj) << (lj + 1)) | (q & (j - 1));
+        const int p = i | j;
+        const uint64_t x = v[i], y = v[p];
+        if ((x > y) == ((i & k2) == 0)) {
+          v[i] = y;
+          v[p] = x;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // The result of signal s: the map (M entries in insertion order) sorted by
 // frequency into the sparse outputs and scattered into the dense one; count
 // and status flags.  All threads; contains barriers.
 __device__ void write_result(const V3Launch &a, int s, const int *keys, const double2 *vals,
                              int M, int flags, double2 *lds)
 {
-
   int P = 1;
   while (P < M) P <<= 1;
   uint64_t *srt = reinterpret_cast<uint64_t *>(lds);
@@ -1063,23 +1105,7 @@ __device__ void write_result(const V3Launch &a, int s, const int *keys, const do
   if (sortable) {
     for (int i = threadIdx.x; i < P; i += blockDim.x)
       srt[i] = i < M ? (((uint64_t)(unsigned)keys[i] << 32) | (unsigned)i) : ~0ull;
-    __syncthreads();
-    for (int k2 = 2; k2 <= P; k2 <<= 1) {
-      for (int j = k2 >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-          const int p = i ^ j;
-          if (p > i) {
-            const uint64_t u = srt[i], v = srt[p];
-            const bool up = (i & k2) == 0;
-            if ((u > v) == up) {
-              srt[i] = v;
-              srt[p] = u;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
+    block_bitonic_u64(srt, P);
   }
   for (int t = threadIdx.x; t < M; t += blockDim.x) {
     const int e = sortable ? (int)(srt[t] & 0xffffffffu) : t;
