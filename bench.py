@@ -383,7 +383,7 @@ def uneven_v12_mix(torch, dev, reps=10):
     (execute_many_sparse of its 4 signals), and checked equal to those
     alone-runs signal for signal."""
     import sfft_amd
-    from sfft_amd.sfft import execute_many_plans
+    from sfft_amd.sfft import ManyPlansCall, execute_many_plans
     from sfft_amd.signals import signals_device
     keys = [(2, 1 << 18, 10), (2, 1 << 18, 50), (2, 1 << 20, 20), (2, 1 << 20, 100),
             (2, 1 << 20, 300), (2, 1 << 22, 10), (2, 1 << 22, 64), (2, 1 << 22, 200),
@@ -401,11 +401,12 @@ def uneven_v12_mix(torch, dev, reps=10):
             order.append(plans[i])
     outs = [(torch.empty(p.capacity, dtype=torch.int32, device=dev),
              torch.empty(p.capacity, dtype=torch.complex128, device=dev)) for p in order]
-    execute_many_plans(order, xs, outs)  # warm-up
+    call = ManyPlansCall(order, xs, outs)  # pointer arrays built once
+    call.call()  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        execute_many_plans(order, xs, outs, trim=False)
+        call.call()
     torch.cuda.synchronize()
     ms_mix = (time.perf_counter() - t0) * 1e3 / reps
     # the same groups alone, and the results of one seeded call of each
@@ -447,7 +448,7 @@ def other_configs(torch, dev):
     the FP64 FFT of the signal; and batched v3, 64 signals of n=2^24 with k
     cycling over {200, 500, 1000} (noiseless) in one multi-plan launch."""
     import sfft_amd
-    from sfft_amd.sfft import execute_many_plans
+    from sfft_amd.sfft import ManyPlansCall, execute_many_plans
     from sfft_amd.signals import signals_device
     out = {}
     for name, (ver, n, k, snr) in {"v1_n20_k50": (1, 1 << 20, 50, None),
@@ -500,12 +501,13 @@ def other_configs(torch, dev):
     order = [plans[ks[i % 3]] for i in range(S)]
     outs = [(torch.empty(p.capacity, dtype=torch.int32, device=dev),
              torch.empty(p.capacity, dtype=torch.complex128, device=dev)) for p in order]
-    execute_many_plans(order, xs, outs)  # warm-up
+    call = ManyPlansCall(order, xs, outs)  # pointer arrays built once
+    call.call()  # warm-up
     torch.cuda.synchronize()
-    reps = 10
+    reps = 20
     t0 = time.perf_counter()
     for _ in range(reps):
-        counts = execute_many_plans(order, xs, outs, trim=False)
+        call.call()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     # oracle parity of the first entry of each plan: the plans' first draws
@@ -549,11 +551,12 @@ def other_configs(torch, dev):
     order_r = [plans_r[ks[i % 3]] for i in range(S)]
     outs_r = [(torch.empty(p.capacity, dtype=torch.int32, device=dev),
                torch.empty(p.capacity, dtype=torch.complex128, device=dev)) for p in order_r]
-    execute_many_plans(order_r, xs, outs_r)  # warm-up
+    call_r = ManyPlansCall(order_r, xs, outs_r)
+    call_r.call()  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        execute_many_plans(order_r, xs, outs_r, trim=False)
+        call_r.call()
     torch.cuda.synchronize()
     dt_r = (time.perf_counter() - t0) / reps
     for kk in ks:

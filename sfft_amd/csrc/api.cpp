@@ -420,11 +420,14 @@ int sfft_exec_many_plans(int num, sfft_plan *const *plans, const complex_t *cons
   int v3rc = 0;
   if (!v3plans.empty()) {
     // one launch on the first v3 plan's stream; it first waits for the work
-    // the caller enqueued on the other v3 plans' streams (their inputs)
+    // the caller enqueued on the other v3 plans' streams (their inputs).  A
+    // v3 plan's private stream holds no work between calls (every v3
+    // exec synchronises it before returning), so it is not waited for.
     hipStream_t st = v3plans[0]->stream;
     std::vector<hipStream_t> waited;
     for (Plan *P : v3plans) {
-      if (P->stream == st || std::find(waited.begin(), waited.end(), P->stream) != waited.end())
+      if (P->stream == st || P->own_stream ||
+          std::find(waited.begin(), waited.end(), P->stream) != waited.end())
         continue;
       waited.push_back(P->stream);
       hipEvent_t ev = nullptr;

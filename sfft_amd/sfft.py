@@ -333,6 +333,45 @@ class sfft:
         return fr[:m].copy(), va[:m].copy()
 
 
+class ManyPlansCall:
+    """A prepared sfft_exec_many_plans over fixed buffers: the plan, input
+    and output pointer arrays are built once, so each call() is the C call
+    alone (for loops that re-run one batch shape, e.g. a serving step or a
+    benchmark).  The tensors must stay allocated while the object is used.
+    call() returns the counts (int64 array) the call wrote."""
+
+    def __init__(self, plans, xs, outputs=None, bind_streams=True):
+        import torch
+        num = len(xs)
+        assert len(plans) == num
+        if bind_streams:
+            stream = torch.cuda.current_stream().cuda_stream
+            for p in {id(p): p for p in plans}.values():
+                p._bind_stream(torch, stream)
+        if outputs is None:
+            outputs = [(torch.empty(p.capacity, dtype=torch.int32, device=x.device),
+                        torch.empty(p.capacity, dtype=torch.complex128, device=x.device))
+                       for p, x in zip(plans, xs)]
+        self.plans, self.xs, self.outputs, self.num = list(plans), list(xs), outputs, num
+        self._pl = np.array([C.cast(p.sfft_plan, C.c_void_p).value for p in plans],
+                            dtype=np.uint64)
+        self._ins = np.array([x.data_ptr() for x in xs], dtype=np.uint64)
+        # sfft_output_c records {count, capacity, freq, val}
+        self._outs = np.zeros((num, 4), dtype=np.uint64)
+        self._outs[:, 1] = [fr.numel() for fr, _ in outputs]
+        self._outs[:, 2] = [fr.data_ptr() for fr, _ in outputs]
+        self._outs[:, 3] = [va.data_ptr() for _, va in outputs]
+        self._fn = _lib.load().sfft_exec_many_plans
+        self._args = (num, self._pl.ctypes.data_as(C.c_void_p),
+                      self._ins.ctypes.data_as(C.c_void_p),
+                      self._outs.ctypes.data_as(C.POINTER(_lib.SfftOutput)))
+
+    def call(self):
+        if self._fn(*self._args) != 0:
+            raise SfftError(_lib.last_error())
+        return self._outs[:, 0].astype(np.int64)
+
+
 def execute_many_plans(plans, xs, outputs=None, trim=True, bind_streams=True):
     """sfft_exec_many_plans: signal i of xs (device tensors) runs plans[i];
     the v3 signals of all plans run as one launch.  outputs: list of
@@ -341,32 +380,9 @@ def execute_many_plans(plans, xs, outputs=None, trim=True, bind_streams=True):
     bind_streams=False on the stream it was given (set_stream).  Returns
     [(freq[:count], val[:count])], or with trim=False the counts (int64
     array) the call wrote, leaving the views to the caller."""
-    import torch
-    num = len(xs)
-    assert len(plans) == num
-    stream = torch.cuda.current_stream().cuda_stream
-    addr = {}
-    for p in {id(p): p for p in plans}.values():
-        if bind_streams:
-            p._bind_stream(torch, stream)
-        addr[id(p)] = C.cast(p.sfft_plan, C.c_void_p).value
-    if outputs is None:
-        outputs = [(torch.empty(p.capacity, dtype=torch.int32, device=x.device),
-                    torch.empty(p.capacity, dtype=torch.complex128, device=x.device))
-                   for p, x in zip(plans, xs)]
-    pl = np.array([addr[id(p)] for p in plans], dtype=np.uint64)
-    ins = np.array([x.data_ptr() for x in xs], dtype=np.uint64)
-    # sfft_output_c records {count, capacity, freq, val}
-    outs = np.zeros((num, 4), dtype=np.uint64)
-    outs[:, 1] = [fr.numel() for fr, _ in outputs]
-    outs[:, 2] = [fr.data_ptr() for fr, _ in outputs]
-    outs[:, 3] = [va.data_ptr() for _, va in outputs]
-    if _lib.load().sfft_exec_many_plans(num, pl.ctypes.data_as(C.c_void_p),
-                                        ins.ctypes.data_as(C.c_void_p),
-                                        outs.ctypes.data_as(C.POINTER(_lib.SfftOutput))) != 0:
-        raise SfftError(_lib.last_error())
-    counts = outs[:, 0].astype(np.int64)
+    m = ManyPlansCall(plans, xs, outputs, bind_streams)
+    counts = m.call()
     if not trim:
         return counts
     return [(fr[:min(int(counts[i]), fr.numel())], va[:min(int(counts[i]), va.numel())])
-            for i, (fr, va) in enumerate(outputs)]
+            for i, (fr, va) in enumerate(m.outputs)]
