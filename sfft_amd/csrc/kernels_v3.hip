@@ -70,7 +70,38 @@ struct Ctx {
   double2 *ev;
   double2 *rem;
   int *remi;
+  // Mansour buckets written since the last Mansour estimate (a bit per
+  // bucket, LDS; null when it does not fit) and the running count of
+  // buckets above the loop's power threshold (LDS)
+  unsigned *dirty;
+  int *cm;
+  double2 *stage;  // LDS for the subtractions, room bytes
+  size_t room;
 };
+
+// The round-robin loop's test on a Mansour bucket (loop-0 value z): power
+// of z / W above 1e-6 (cf3 1054-1060).  W is a power of two, so z * (1 / W)
+// rounds like z / W.
+__device__ __forceinline__ int ms_counted(double2 z, double invW)
+{
+  const double re = z.x * invW, im = z.y * invW;
+  return re * re + im * im > 1e-6;
+}
+
+// MS bucket h less (r0, r1), its dirty bit set and the count updated.
+__device__ __forceinline__ void ms_sub(const Ctx &c, int h, double2 r0, double2 r1)
+{
+  const double2 old = c.MS[2 * h];
+  const double2 now = csub(old, r0);
+  c.MS[2 * h] = now;
+  c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r1);
+  if (c.dirty) {
+    const double invW = 1.0 / c.W;
+    atomicOr(&c.dirty[h >> 5], 1u << (h & 31));
+    const int d = ms_counted(now, invW) - ms_counted(old, invW);
+    if (d) atomicAdd(c.cm, d);
+  }
+}
 
 // ---------------------------------------------------------------- map
 __device__ __forceinline__ unsigned hslot(int key, int hcap)
@@ -311,11 +342,9 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
                                          int cnt, int which)
 {
   const V3Launch &a = *c.a;
-  extern __shared__ double2 lds[];
-  // LDS past the four Gauss bucket arrays (staging for the applications)
-  const size_t used = (2 * (size_t)c.B1 + 2 * (size_t)c.B2) * sizeof(double2);
-  const size_t room = a.lds_bytes > used ? a.lds_bytes - used : 0;
-  double2 *const sv = lds + 2 * c.B1 + 2 * c.B2;
+  // LDS staging for the applications
+  const size_t room = c.room;
+  double2 *const sv = c.stage;
   if (cnt <= kSerialApply && !a.serial_all &&
       room >= (size_t)kSerialApply * (14 * sizeof(double2) + 3 * sizeof(int))) {
     // Few pairs: the removal values stay in LDS; one thread per Gauss
@@ -345,8 +374,7 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
       for (int i = 0; i < cnt; i++) {
         const int h = si[i * 3 + 2];
         if ((h & (kV3Block - 1)) != (int)threadIdx.x) continue;
-        c.MS[2 * h] = csub(c.MS[2 * h], sv[i * 14 + 12]);
-        c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], sv[i * 14 + 13]);
+        ms_sub(c, h, sv[i * 14 + 12], sv[i * 14 + 13]);
       }
     }
     __threadfence_block();
@@ -380,8 +408,7 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
       for (int i = 0; i < cnt; i++) {
         const int h = c.remi[i * 3 + 2];
         const double2 *r = c.rem + (int64_t)i * 14 + 12;
-        c.MS[2 * h] = csub(c.MS[2 * h], r[0]);
-        c.MS[2 * h + 1] = csub(c.MS[2 * h + 1], r[1]);
+        ms_sub(c, h, r[0], r[1]);
       }
     }
     __threadfence_block();
@@ -406,7 +433,6 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
     const int t = threadIdx.x;
     const int *__restrict__ remi = c.remi;
     const double2 *__restrict__ rem = c.rem;
-    double2 *__restrict__ ms = c.MS;
     for (int i0 = 0; i0 < cnt; i0 += kApplyAhead) {
       int hh[kApplyAhead];
 #pragma unroll
@@ -418,8 +444,7 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
         if (h < 0) break;
         if ((h & (T - 1)) != t) continue;  // h = key mod W >= 0
         const double2 *r = rem + (int64_t)(i0 + u) * 14 + 12;
-        ms[2 * h] = csub(ms[2 * h], r[0]);
-        ms[2 * h + 1] = csub(ms[2 * h + 1], r[1]);
+        ms_sub(c, h, r[0], r[1]);
       }
     }
   }
@@ -623,6 +648,58 @@ __device__ __forceinline__ int est_mansour(const Ctx &c, int *rank, int *sh)
     }
     found += total;
     __syncthreads();  // rank is rewritten by the next segment
+  }
+  __threadfence_block();
+  __syncthreads();
+  return found;
+}
+
+// est_mansour over the buckets written since the previous Mansour estimate
+// only: a bucket left unwritten holds the values that estimate rejected (an
+// accepted bucket is always written next, by the subtraction of what it
+// found), so it is rejected again and the result is the full sweep's.
+// Thread t takes bit word w = t (+ kV3Block ...): buckets 32 w .. 32 w + 31
+// in ascending order, so a block scan over the threads' accepted counts
+// ranks the accepted buckets in bucket order.  Clears the bits it reads.
+__device__ __forceinline__ int est_mansour_dirty(const Ctx &c, int *sh)
+{
+  const int n = c.n, W = c.W;
+  const double PI2 = 2 * M_PI;
+  MansourConst m;
+  m.N_OVER_PI2 = n / PI2;
+  m.PI2_OVER_N = PI2 / n;
+  m.FREQ_MASK = ((unsigned)n - 1) & ~((unsigned)W - 1);
+  m.NORM = 1. / W;
+  m.NORM2 = m.NORM * m.NORM;
+  m.err_max = n * 1e-10;
+  const int words = (W + 31) >> 5;
+  int found = 0;
+  for (int w0 = 0; w0 < words; w0 += blockDim.x) {
+    const int wd = w0 + threadIdx.x;
+    unsigned bits = 0, acc = 0;
+    if (wd < words) {
+      bits = c.dirty[wd];
+      c.dirty[wd] = 0u;
+    }
+    for (unsigned b = bits; b; b &= b - 1) {
+      const int bk = wd * 32 + __ffs(b) - 1;
+      int o;
+      double2 v;
+      if (mansour_bucket<false>(c, m, bk, c.MS[2 * bk], c.MS[2 * bk + 1], o, v))
+        acc |= b & (~b + 1u);
+    }
+    int total;
+    int pos = found + block_exclusive_scan(__popc(acc), sh, total);
+    for (; acc; acc &= acc - 1) {
+      const int bk = wd * 32 + __ffs(acc) - 1;
+      int o = -1;
+      double2 v = make_double2(0, 0);
+      (void)mansour_bucket<true>(c, m, bk, c.MS[2 * bk], c.MS[2 * bk + 1], o, v);
+      c.ef[pos] = o;
+      c.ev[pos] = v;
+      pos++;
+    }
+    found += total;
   }
   __threadfence_block();
   __syncthreads();
@@ -898,7 +975,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   extern __shared__ double2 lds[];
   __shared__ int sh[16];
   __shared__ int s_rank[kV3Block];
-  __shared__ int s_size, s_flag, s_cnt[3];
+  __shared__ int s_size, s_flag, s_cnt[3], s_cm;
   Ctx c;
   c.a = &a;
   c.s = s;
@@ -928,6 +1005,20 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   c.rem = a.rem + (int64_t)s * a.Rcap * 14;
   c.remi = a.remi + (int64_t)s * a.Rcap * 3;
   const int W = a.W;
+  {
+    // LDS past the four Gauss bucket arrays: the dirty bits (when they
+    // leave room for the subtractions' smallest staging), then staging
+    const size_t gauss = (2 * (size_t)a.B1 + 2 * (size_t)a.B2) * sizeof(double2);
+    const size_t bits = (((size_t)W + 31) / 32 * sizeof(unsigned) + 15) / 16 * 16;
+    const size_t rest = a.lds_bytes > gauss ? a.lds_bytes - gauss : 0;
+    const bool track = rest >= bits + (size_t)kMinStageChunk * 112 + 4 * ((size_t)max(a.B1, a.B2) + 16);
+    c.dirty = track ? reinterpret_cast<unsigned *>(lds + 2 * a.B1 + 2 * a.B2) : nullptr;
+    c.cm = &s_cm;
+    c.stage = lds + 2 * a.B1 + 2 * a.B2 + (track ? bits / sizeof(double2) : 0);
+    c.room = rest - (track ? bits : 0);
+    if (track)
+      for (int i = threadIdx.x; i < (W + 31) / 32; i += blockDim.x) c.dirty[i] = 0u;
+  }
   for (int i = threadIdx.x; i < a.Hcap; i += blockDim.x) c.hidx[i] = -1;
   if (threadIdx.x == 0) {
     s_size = 0;
@@ -942,9 +1033,14 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
     if (threadIdx.x == 0) s_flag |= 2;
   }
   for (int i = threadIdx.x; i < F; i += blockDim.x) {
-    const int h = c.ef[i] % W;
+    const int h = c.ef[i] % W;  // one found per bucket
+    const double2 old = c.MS[2 * h];
     c.MS[2 * h] = make_double2(0, 0);
     c.MS[2 * h + 1] = make_double2(0, 0);
+    if (c.dirty) {
+      atomicOr(&c.dirty[h >> 5], 1u << (h & 31));
+      if (ms_counted(old, 1.0 / W)) atomicAdd(c.cm, -1);
+    }
   }
   __threadfence_block();
   __syncthreads();
@@ -989,9 +1085,10 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   // ---- round robin until the three bucket counts repeat (cf3 997-1076)
   int prev[3] = {0, 0, 0};
   int pass = 0;
+  bool cm_valid = false;  // s_cm holds the Mansour count (uniform)
   for (; pass < a.max_passes; pass++) {
     const int sel = pass % 3;
-    if (sel == 0) F = est_mansour(c, s_rank, sh);
+    if (sel == 0) F = c.dirty ? est_mansour_dirty(c, sh) : est_mansour(c, s_rank, sh);
     else if (sel == 1) F = est_gauss(c, c.GS0, c.GS1, a.B1, 1, 0, a.win1, a.win1_half, sh);
     else F = est_gauss(c, c.GP0, c.GP1, a.B2, c.aa, c.b, a.win2, a.win2_half, sh);
     if (!map_add_found(c, F, sh, &s_size)) {
@@ -1013,24 +1110,28 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
         const double2 z = (j & 1) ? c.GP1[j >> 1] : c.GP0[j >> 1];
         c2 += z.x * z.x + z.y * z.y > 1e-6;
       }
-      // W is a power of two: z / W and z * (1 / W) round the same value
-      const double invW = 1.0 / W;
-      for (int j0 = threadIdx.x; j0 < W; j0 += kSweepAhead * kV3Block) {
-        double2 z[kSweepAhead];  // loaded together, clamped past the end
+      // the Mansour count is kept up to date by every bucket write once it
+      // has been counted in full (the first test)
+      if (!cm_valid) {
+        const double invW = 1.0 / W;
+        for (int j0 = threadIdx.x; j0 < W; j0 += kSweepAhead * kV3Block) {
+          double2 z[kSweepAhead];  // loaded together, clamped past the end
 #pragma unroll
-        for (int u = 0; u < kSweepAhead; u++) z[u] = c.MS[2 * min(j0 + u * kV3Block, W - 1)];
+          for (int u = 0; u < kSweepAhead; u++) z[u] = c.MS[2 * min(j0 + u * kV3Block, W - 1)];
 #pragma unroll
-        for (int u = 0; u < kSweepAhead; u++) {
-          const double re = z[u].x * invW, im = z[u].y * invW;
-          cm += j0 + u * kV3Block < W && re * re + im * im > 1e-6;
+          for (int u = 0; u < kSweepAhead; u++) cm += j0 + u * kV3Block < W && ms_counted(z[u], invW);
         }
       }
-      atomicAdd(&s_cnt[0], cm);
+      if (!cm_valid) atomicAdd(&s_cnt[0], cm);
       atomicAdd(&s_cnt[1], c1);
       atomicAdd(&s_cnt[2], c2);
       __syncthreads();
-      const int now[3] = {s_cnt[0], s_cnt[1], s_cnt[2]};
+      const int now[3] = {cm_valid ? s_cm : s_cnt[0], s_cnt[1], s_cnt[2]};
       __syncthreads();
+      if (!cm_valid && c.dirty) {
+        if (threadIdx.x == 0) s_cm = now[0];
+        cm_valid = true;
+      }
       if (now[0] == prev[0] && now[1] == prev[1] && now[2] == prev[2]) break;
       prev[0] = now[0];
       prev[1] = now[1];
