@@ -38,6 +38,14 @@ constexpr int kStageChunk = 1024, kMinStageChunk = 32;
 
 __device__ __forceinline__ int ilog2v(int v) { return 31 - __clz(v); }
 
+// Sum over the 64 lanes of a wave (every lane gets it).
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // int products in the reference wrap modulo 2^32 (cf3 381-382, 475, 743)
 __device__ __forceinline__ int wmul(int x, int y) { return (int)((unsigned)x * (unsigned)y); }
 
@@ -157,19 +165,36 @@ __device__ void fft_pair(double2 *x0, double2 *x1, int B, const V3Launch &a)
 // ---------------------------------------------------------------- removals
 // update_gaussian_loops2 (cf3 354-463): what to subtract from buckets
 // h, h+1, h-1 of both loops for (key, value); r[0..5] = loop0 x3, loop1 x3.
-__device__ void gauss_removal(const Ctx &c, int key, double2 value, int B,
-                              const double2 *win, int half, int g, int step,
-                              int &h, double2 *r)
+// update_gauss_loops2 (cf3 353-463) for one located (key, value) in a
+// Gauss filter of B buckets at offsets g, g + step: the centre bucket h, the
+// three window distances, then (gauss_values) the removal values r[0..2]
+// (offset g) and r[3..5] (g + step) for buckets h, h+1, h-1.  Split so the
+// window loads of several removals can be issued together.
+struct GaussIdx {
+  int h;
+  unsigned dd[3];
+};
+
+__device__ __forceinline__ GaussIdx gauss_index(const Ctx &c, int key, int B)
 {
   const int n = c.n;
   const unsigned n1 = (unsigned)n - 1;
-  const double PI2_DIV_N = 2 * M_PI / n;
   const unsigned nb = (unsigned)n / (unsigned)B;
-  h = (int)(key * 1. / nb + 0.5) % B;
+  GaussIdx x;
+  x.h = (int)(key * 1. / nb + 0.5) % B;
+  const unsigned dist1 = ((unsigned)x.h * nb - (unsigned)key + (unsigned)n) & n1;
+  x.dd[0] = dist1;
+  x.dd[1] = (dist1 + (unsigned)n + nb) & n1;
+  x.dd[2] = (dist1 + (unsigned)n + (unsigned)n - nb) & n1;
+  return x;
+}
+
+__device__ __forceinline__ void gauss_values(const Ctx &c, int key, double2 value, int g,
+                                             int step, const double2 F[3], double2 *r)
+{
+  const int n = c.n;
+  const double PI2_DIV_N = 2 * M_PI / n;
   const double a1 = value.x, b1 = value.y;
-  const unsigned dist1 = ((unsigned)h * nb - (unsigned)key + (unsigned)n) & n1;
-  const unsigned dist2 = (dist1 + (unsigned)n + nb) & n1;
-  const unsigned dist3 = (dist1 + (unsigned)n + (unsigned)n - nb) & n1;
   const int ko = wmul(key, g) % n;
   const int ko2 = wmul(key, (int)((unsigned)g + (unsigned)step)) % n;
   double b2, a2, b22, a22;
@@ -177,13 +202,23 @@ __device__ void gauss_removal(const Ctx &c, int key, double2 value, int B,
   sincos(PI2_DIV_N * ko2, &b22, &a22);
   const double v1r = a1 * a2 - b1 * b2, v1i = a1 * b2 + b1 * a2;
   const double v2r = a1 * a22 - b1 * b22, v2i = a1 * b22 + b1 * a22;
-  const unsigned dd[3] = {dist1, dist2, dist3};
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    const double2 F = win_at(win, half, dd[q], n);
-    r[q] = make_double2(v1r * F.x - v1i * F.y, v1r * F.y + v1i * F.x);
-    r[3 + q] = make_double2(v2r * F.x - v2i * F.y, v2r * F.y + v2i * F.x);
+    r[q] = make_double2(v1r * F[q].x - v1i * F[q].y, v1r * F[q].y + v1i * F[q].x);
+    r[3 + q] = make_double2(v2r * F[q].x - v2i * F[q].y, v2r * F[q].y + v2i * F[q].x);
   }
+}
+
+__device__ void gauss_removal(const Ctx &c, int key, double2 value, int B,
+                              const double2 *win, int half, int g, int step,
+                              int &h, double2 *r)
+{
+  const GaussIdx x = gauss_index(c, key, B);
+  h = x.h;
+  double2 F[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) F[q] = win_at(win, half, x.dd[q], c.n);
+  gauss_values(c, key, value, g, step, F, r);
 }
 
 __device__ void gauss_apply_pair(double2 *S0, double2 *S1, int B, int h, const double2 *r)
@@ -191,6 +226,47 @@ __device__ void gauss_apply_pair(double2 *S0, double2 *S1, int B, int h, const d
   const int bk[3] = {h, (h + 1) % B, (h + B - 1) % B};
   for (int q = 0; q < 3; q++) S0[bk[q]] = csub(S0[bk[q]], r[q]);
   for (int q = 0; q < 3; q++) S1[bk[q]] = csub(S1[bk[q]], r[3 + q]);
+}
+
+// A few pairs (removal values in LDS: 14 per pair, this filter's at roff;
+// bucket index at 3 i + slot) applied by bucket owners: thread t takes its
+// buckets t, t + blockDim, ... and walks the pairs in order, applying the
+// parts that land on the bucket (h, h+1, h-1 within a pair) -- the order
+// in which gauss_apply_pair applies them pair after pair.
+__device__ __forceinline__ void gauss_apply_few(double2 *S0, double2 *S1, int B, const int *si,
+                                                int slot, const double2 *sv, int roff, int cnt)
+{
+  const int bm = B - 1;  // B a power of two
+  for (int tb = threadIdx.x; tb < B; tb += blockDim.x) {
+    double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+    bool any = false;
+    for (int i = 0; i < cnt; i++) {
+      const int h = si[i * 3 + slot];
+      if ((unsigned)((tb - h + 1) & bm) > 2u) continue;
+      if (!any) {
+        a0 = S0[tb];
+        a1 = S1[tb];
+        any = true;
+      }
+      const double2 *r = sv + i * 14 + roff;
+      if (h == tb) {
+        a0 = csub(a0, r[0]);
+        a1 = csub(a1, r[3]);
+      }
+      if (((h + 1) & bm) == tb) {
+        a0 = csub(a0, r[1]);
+        a1 = csub(a1, r[4]);
+      }
+      if (((h + bm) & bm) == tb) {
+        a0 = csub(a0, r[2]);
+        a1 = csub(a1, r[5]);
+      }
+    }
+    if (any) {
+      S0[tb] = a0;
+      S1[tb] = a1;
+    }
+  }
 }
 
 // Pair i subtracts r[0..2] from buckets h, h+1, h-1 of S0 and r[3..5] from
@@ -357,19 +433,30 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
       const double2 v = vals[i];
       double2 *r = sv + i * 14;
       int *ri = si + i * 3;
-      if (which & 1) gauss_removal(c, key, v, c.B1, a.win1, a.win1_half, c.gG, 1, ri[0], r);
-      if (which & 2)
-        gauss_removal(c, (timesmod(key, c.ai, c.n) + c.shift) % c.n, v, c.B2, a.win2,
-                      a.win2_half, wmul(c.aa, c.gG), 1, ri[1], r + 6);
+      // both filters' window values loaded together
+      const int key2 = (timesmod(key, c.ai, c.n) + c.shift) % c.n;
+      GaussIdx x1 = {}, x2 = {};
+      double2 F1[3], F2[3];
+      if (which & 1) x1 = gauss_index(c, key, c.B1);
+      if (which & 2) x2 = gauss_index(c, key2, c.B2);
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        if (which & 1) F1[q] = win_at(a.win1, a.win1_half, x1.dd[q], c.n);
+        if (which & 2) F2[q] = win_at(a.win2, a.win2_half, x2.dd[q], c.n);
+      }
+      if (which & 1) {
+        ri[0] = x1.h;
+        gauss_values(c, key, v, c.gG, 1, F1, r);
+      }
+      if (which & 2) {
+        ri[1] = x2.h;
+        gauss_values(c, key2, v, wmul(c.aa, c.gG), 1, F2, r + 6);
+      }
       if (which & 4) mansour_removal(c, key, v, ri[2], r + 12);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && (which & 1)) {
-      for (int i = 0; i < cnt; i++) gauss_apply_pair(c.GS0, c.GS1, c.B1, si[i * 3], sv + i * 14);
-    } else if (threadIdx.x == 64 && (which & 2)) {
-      for (int i = 0; i < cnt; i++)
-        gauss_apply_pair(c.GP0, c.GP1, c.B2, si[i * 3 + 1], sv + i * 14 + 6);
-    }
+    if (which & 1) gauss_apply_few(c.GS0, c.GS1, c.B1, si, 0, sv, 0, cnt);
+    if (which & 2) gauss_apply_few(c.GP0, c.GP1, c.B2, si, 1, sv, 6, cnt);
     if (which & 4) {
       for (int i = 0; i < cnt; i++) {
         const int h = si[i * 3 + 2];
@@ -1122,9 +1209,14 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
           for (int u = 0; u < kSweepAhead; u++) cm += j0 + u * kV3Block < W && ms_counted(z[u], invW);
         }
       }
-      if (!cm_valid) atomicAdd(&s_cnt[0], cm);
-      atomicAdd(&s_cnt[1], c1);
-      atomicAdd(&s_cnt[2], c2);
+      c1 = wave_sum(c1);
+      c2 = wave_sum(c2);
+      if (!cm_valid) cm = wave_sum(cm);
+      if ((threadIdx.x & 63) == 0) {
+        if (!cm_valid) atomicAdd(&s_cnt[0], cm);
+        atomicAdd(&s_cnt[1], c1);
+        atomicAdd(&s_cnt[2], c2);
+      }
       __syncthreads();
       const int now[3] = {cm_valid ? s_cm : s_cnt[0], s_cnt[1], s_cnt[2]};
       __syncthreads();
