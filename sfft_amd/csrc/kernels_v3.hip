@@ -32,6 +32,8 @@ constexpr int kSerialApply = 16;
 constexpr int kApplyAhead = 32;
 // Steps of a sweep over the Mansour buckets whose loads are issued together.
 constexpr int kSweepAhead = 4;
+// Loads the pre-kernel's gathers and folds issue together per thread.
+constexpr int kPreAhead = 8;
 // Pairs per LDS chunk of the Gauss subtractions (fewer fitting than
 // kMinStageChunk: indices and values read from memory instead).
 constexpr int kStageChunk = 1024, kMinStageChunk = 32;
@@ -1318,6 +1320,26 @@ __device__ void write_result(const V3Launch &a, int s, const int *keys, const do
   }
 }
 
+// lds[bitrev(i)] = x[(off + i sigma) & mask] for i < W (the input of an
+// in-place W-point FFT), kPreAhead loads per thread issued together.
+__device__ __forceinline__ void gather_strided(double2 *lds, const double2 *x, unsigned off,
+                                               int sigma, int W, int logW, unsigned mask)
+{
+  const int T = blockDim.x;
+  for (int i0 = threadIdx.x; i0 < W; i0 += kPreAhead * T) {
+    double vr[kPreAhead], vi[kPreAhead];  // (scalars: a double2 array went to scratch)
+#pragma unroll
+    for (int u = 0; u < kPreAhead; u++) {
+      const double2 v = x[(off + (unsigned)min(i0 + u * T, W - 1) * (unsigned)sigma) & mask];
+      vr[u] = v.x;
+      vi[u] = v.y;
+    }
+#pragma unroll
+    for (int u = 0; u < kPreAhead; u++)  // past the end: sample W - 1 again, same value
+      lds[bitrev((unsigned)min(i0 + u * T, W - 1), logW)] = make_double2(vr[u], vi[u]);
+  }
+}
+
 // The start of every signal's transform that does not depend on its map of
 // located frequencies, batch-wide, four workgroups per signal (the batch's
 // other CUs would idle while one workgroup per signal did it in turn):
@@ -1353,8 +1375,7 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
     if (!a.refine || a.ms_pre) return;
     const int l = task - 4, sigma = n / W, logW = ilog2v(W);
     const unsigned off = (unsigned)g0 + (unsigned)l * (unsigned)a.ref_t;
-    for (int i = threadIdx.x; i < W; i += blockDim.x)
-      lds[bitrev((unsigned)i, logW)] = x[(off + (unsigned)i * (unsigned)sigma) & mask];
+    gather_strided(lds, x, off, sigma, W, logW, mask);
     __syncthreads();
     lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
     double2 *RM = a.ref_ms + ((int64_t)s * kRefOff + l) * W;
@@ -1371,9 +1392,7 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
       return;
     }
     const int l = task, sigma = n / W, logW = ilog2v(W);
-    for (int i = threadIdx.x; i < W; i += blockDim.x)
-      lds[bitrev((unsigned)i, logW)] =
-          x[((unsigned)g0 + (unsigned)l + (unsigned)i * (unsigned)sigma) & mask];
+    gather_strided(lds, x, (unsigned)g0 + (unsigned)l, sigma, W, logW, mask);
     __syncthreads();
     lds_fft(lds, logW, a.tw64, a.tw_nmax / W);
     for (int i = threadIdx.x; i < W; i += blockDim.x) MS[2 * i + l] = lds[i];
@@ -1384,15 +1403,25 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
     double2 *GS0 = lds, *GS1 = lds + B;
     for (int r = threadIdx.x; r < B; r += blockDim.x) {
       double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
-      for (int ch = 0; ch < chunks; ch++) {
-        const int t = ch * B + r;
-        const double2 tp = a.taps1[t];
-        const double2 x0 = x[((unsigned)gG + (unsigned)t) & mask];
-        const double2 x1 = x[((unsigned)gG + 1u + (unsigned)t) & mask];
-        s0.x = s0.x + (tp.x * x0.x - tp.y * x0.y);
-        s0.y = s0.y + (tp.y * x0.x + tp.x * x0.y);
-        s1.x = s1.x + (tp.x * x1.x - tp.y * x1.y);
-        s1.y = s1.y + (tp.y * x1.x + tp.x * x1.y);
+      // kPreAhead chunks' loads issued together (clamped to the last
+      // chunk), then summed in chunk order
+      for (int c0 = 0; c0 < chunks; c0 += kPreAhead) {
+        double2 tp[kPreAhead], x0[kPreAhead], x1[kPreAhead];
+#pragma unroll
+        for (int u = 0; u < kPreAhead; u++) {
+          const int t = min(c0 + u, chunks - 1) * B + r;
+          tp[u] = a.taps1[t];
+          x0[u] = x[((unsigned)gG + (unsigned)t) & mask];
+          x1[u] = x[((unsigned)gG + 1u + (unsigned)t) & mask];
+        }
+#pragma unroll
+        for (int u = 0; u < kPreAhead; u++) {
+          if (c0 + u >= chunks) break;
+          s0.x = s0.x + (tp[u].x * x0[u].x - tp[u].y * x0[u].y);
+          s0.y = s0.y + (tp[u].y * x0[u].x + tp[u].x * x0[u].y);
+          s1.x = s1.x + (tp[u].x * x1[u].x - tp[u].y * x1[u].y);
+          s1.y = s1.y + (tp[u].y * x1[u].x + tp[u].x * x1[u].y);
+        }
       }
       GS0[r] = s0;
       GS1[r] = s1;
@@ -1415,6 +1444,28 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
   double2 *phase = (2 * (size_t)B + a.w2 + 1) * sizeof(double2) <= a.lds_bytes
                        ? lds + 2 * B
                        : a.phase + (int64_t)s * a.Pcap;
+  // while lane 0 runs the chain, waves 1.. gather the permuted samples
+  // x[(gG + i ai) mod n], i <= w2 (cf3 78-85; gG, ai >= 0 and n a power of
+  // two, so the mod is a mask) into LDS past the phases, where they fit
+  const bool staged =
+      (2 * (size_t)B + 2 * ((size_t)a.w2 + 1)) * sizeof(double2) <= a.lds_bytes;
+  double2 *xs = lds + 2 * B + a.w2 + 1;  // (an LDS address either way)
+  if (staged && threadIdx.x >= 64) {
+    const int T1 = blockDim.x - 64;
+    for (int i0 = threadIdx.x - 64; i0 <= a.w2; i0 += kPreAhead * T1) {
+      double vr[kPreAhead], vi[kPreAhead];  // (scalars: a double2 array went to scratch)
+#pragma unroll
+      for (int u = 0; u < kPreAhead; u++) {
+        const unsigned long long i = (unsigned long long)min(i0 + u * T1, a.w2);
+        const double2 v = x[((unsigned long long)gG + i * (unsigned long long)ai) & (unsigned long long)mask];
+        vr[u] = v.x;
+        vi[u] = v.y;
+      }
+#pragma unroll
+      for (int u = 0; u < kPreAhead; u++)  // past the end: sample w2 again, same value
+        xs[min(i0 + u * T1, a.w2)] = make_double2(vr[u], vi[u]);
+    }
+  }
   if (threadIdx.x == 0) {
     double s0, c0, s1, c1;
     sincos(2 * M_PI * gG * b / n, &s0, &c0);
@@ -1439,8 +1490,9 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
 #pragma unroll
       for (int q = 0; q < 2; q++) {
         // sample (gG + (t+q) ai) mod n (cf3 78-85)
-        const unsigned idx = (unsigned)(((long long)gG + (long long)(t + q) * ai) % n);
-        const double2 xv = x[idx], sv = phase[t + q];
+        const double2 xv =
+            staged ? xs[t + q] : x[(unsigned)(((long long)gG + (long long)(t + q) * ai) % n)];
+        const double2 sv = phase[t + q];
         p[q] = make_double2(xv.x * sv.x - xv.y * sv.y, xv.x * sv.y + xv.y * sv.x);
       }
       s0.x = s0.x + (tp.x * p[0].x - tp.y * p[0].y);
