@@ -1320,6 +1320,18 @@ __device__ void write_result(const V3Launch &a, int s, const int *keys, const do
   }
 }
 
+// permute_fourier's phases (cf3 66-87): phase[i] = shift step^i for
+// i <= w by repeated complex multiplication, the reference's rounding.
+__device__ __forceinline__ void phase_chain(double2 *phase, double2 sh, const double2 st, int w)
+{
+  // unrolled: the loop control and stores stay off the product chain
+#pragma unroll 8
+  for (int i = 0; i <= w; i++) {
+    phase[i] = sh;
+    sh = make_double2(sh.x * st.x - sh.y * st.y, sh.x * st.y + sh.y * st.x);
+  }
+}
+
 // lds[bitrev(i)] = x[(off + i sigma) & mask] for i < W (the input of an
 // in-place W-point FFT), kPreAhead loads per thread issued together.
 __device__ __forceinline__ void gather_strided(double2 *lds, const double2 *x, unsigned off,
@@ -1441,9 +1453,9 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
   // when it fits: a lone lane's global stores are several times slower)
   const int B = a.B2, chunks = a.w2 / B;
   double2 *GP0 = lds, *GP1 = lds + B;
-  double2 *phase = (2 * (size_t)B + a.w2 + 1) * sizeof(double2) <= a.lds_bytes
-                       ? lds + 2 * B
-                       : a.phase + (int64_t)s * a.Pcap;
+  const bool ph_lds = (2 * (size_t)B + a.w2 + 1) * sizeof(double2) <= a.lds_bytes;
+  double2 *const ph_s = lds + 2 * B;                     // LDS (typed stores)
+  double2 *const ph_g = a.phase + (int64_t)s * a.Pcap;  // HBM otherwise
   // while lane 0 runs the chain, waves 1.. gather the permuted samples
   // x[(gG + i ai) mod n], i <= w2 (cf3 78-85; gG, ai >= 0 and n a power of
   // two, so the mod is a mask) into LDS past the phases, where they fit
@@ -1470,14 +1482,12 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
     double s0, c0, s1, c1;
     sincos(2 * M_PI * gG * b / n, &s0, &c0);
     sincos(2 * M_PI * ai * b / n, &s1, &c1);
-    double2 sh = make_double2(c0, s0);
+    const double2 sh = make_double2(c0, s0);
     const double2 st = make_double2(c1, s1);
-    // unrolled: the loop control and stores stay off the product chain
-#pragma unroll 8
-    for (int i = 0; i <= a.w2; i++) {
-      phase[i] = sh;
-      sh = make_double2(sh.x * st.x - sh.y * st.y, sh.x * st.y + sh.y * st.x);
-    }
+    // separate calls so the LDS stores compile as LDS stores (a pointer
+    // that may be either takes the slower generic path)
+    if (ph_lds) phase_chain(ph_s, sh, st, a.w2);
+    else phase_chain(ph_g, sh, st, a.w2);
   }
   __threadfence_block();
   __syncthreads();
@@ -1492,7 +1502,7 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
         // sample (gG + (t+q) ai) mod n (cf3 78-85)
         const double2 xv =
             staged ? xs[t + q] : x[(unsigned)(((long long)gG + (long long)(t + q) * ai) % n)];
-        const double2 sv = phase[t + q];
+        const double2 sv = ph_lds ? ph_s[t + q] : ph_g[t + q];
         p[q] = make_double2(xv.x * sv.x - xv.y * sv.y, xv.x * sv.y + xv.y * sv.x);
       }
       s0.x = s0.x + (tp.x * p[0].x - tp.y * p[0].y);
