@@ -767,12 +767,13 @@ int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Works
   if (blocks.empty()) return 0;
   const size_t desc_bytes = al256((size_t)G * sizeof(k::V3Launch));
   const size_t blk_bytes = al256(blocks.size() * sizeof(int2));
-  const size_t up_total = desc_bytes + blk_bytes + up;
+  const size_t clm_bytes = al256(blocks.size() * 4 * sizeof(int));  // zeroed claim words
+  const size_t up_total = desc_bytes + blk_bytes + clm_bytes + up;
   if (!ws->multi.ensure(up_total + dn) || ensure_pinned_ws(ws, up_total + dn) != 0) return -1;
   V3Stage sg;
   sg.h = reinterpret_cast<char *>(ws->pinned);
   sg.d = reinterpret_cast<char *>(ws->multi.p);
-  sg.up = desc_bytes + blk_bytes;
+  sg.up = desc_bytes + blk_bytes + clm_bytes;
   sg.dn = up_total;
   for (int g = 0; g < G; g++) {
     if (v3_prepare(plans[g], ios[g], pd[g], sg) != 0) return -1;
@@ -780,6 +781,7 @@ int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Works
   }
   std::memcpy(sg.h, groups.data(), (size_t)G * sizeof(k::V3Launch));
   std::memcpy(sg.h + desc_bytes, blocks.data(), blocks.size() * sizeof(int2));
+  std::memset(sg.h + desc_bytes + blk_bytes, 0, clm_bytes);
   HIP_OK(hipMemcpyAsync(sg.d, sg.h, up_total, hipMemcpyHostToDevice, st));
   const bool timing = timer && timer->timing;
   if (timing && !timer->ev_made) {
@@ -788,7 +790,8 @@ int exec_multi_v3(Plan *const *plans, BatchIO *ios, int G, hipStream_t st, Works
   }
   if (timing) (void)hipEventRecord(timer->ev[0], st);
   if (k::v3_multi(groups.data(), G, reinterpret_cast<k::V3Launch *>(sg.d),
-                  reinterpret_cast<int2 *>(sg.d + desc_bytes), (int)blocks.size(), st) != 0)
+                  reinterpret_cast<int2 *>(sg.d + desc_bytes), (int)blocks.size(),
+                  reinterpret_cast<int *>(sg.d + desc_bytes + blk_bytes), st) != 0)
     return -1;
   if (timing) (void)hipEventRecord(timer->ev[1], st);
   HIP_OK(hipMemcpyAsync(sg.h + up_total, sg.d + up_total, dn, hipMemcpyDeviceToHost, st));

@@ -387,7 +387,7 @@ struct V3Launch {
   // per-signal workspace, strided by signal
   double2 *ms;               // 2W (interleaved loops)
   double2 *ms_pre, *ms_tmp;  // W above the LDS size: S x ms_loops x W
-                             // transforms computed before k_v3_multi (else null)
+                             // transforms computed before k_v3_run (else null)
   int ms_loops;              // Mansour offsets in ms_pre: the reference's
                              // two (init_offset + 0, 1), then with refine
                              // init_offset + l ref_t, 1 <= l < kV3RefineOffsets
@@ -397,7 +397,7 @@ struct V3Launch {
   // ref_t is odd, near 0.618 n/W: members of a bucket m W apart sit at
   // phase steps m ref_t / (n/W) mod 1, well apart for the small m at which
   // the reference places its false positives; ref_tinv = ref_t^-1 mod n/W.
-  double2 *gpre;             // Gauss buckets from k_v3_pre: S x (2 B1 + 2 B2)
+  double2 *gpre;             // Gauss buckets from the pre tasks: S x (2 B1 + 2 B2)
   int refine;
   int ref_t, ref_tinv;
   int serial_all;            // every subtraction pass serial (diagnostic)
@@ -418,7 +418,8 @@ struct V3Launch {
   int max_passes;
   int64_t *count;            // S located counts
   int *status;               // S: 1 early return, 2 map full, 4 pass cap,
-                             // 8 refine kept some reference values
+                             // 8 refine kept some reference values,
+                             // 16 gave up waiting for a pre task (a defect)
   int32_t *const *o_freq;
   double2 *const *o_val;
   double2 *const *o_dense;
@@ -426,9 +427,10 @@ struct V3Launch {
   size_t lds_bytes;
 };
 // G groups in one launch; d_groups / d_blocks: the descriptors and the
-// nblocks (group, signal) pairs, already on the device (enqueued on st).
+// nblocks (group, signal) pairs, d_claims 4 nblocks zeroed ints, already on
+// the device (enqueued on st).
 int v3_multi(const V3Launch *h_groups, int G, const V3Launch *d_groups, const int2 *d_blocks,
-             int nblocks, hipStream_t st);
+             int nblocks, int *d_claims, hipStream_t st);
 
 // Batched power-of-two FFTs; transform t = (outer, inner) lives at
 // data + outer*so + inner*si.  In LDS for small forward sizes (tw: the

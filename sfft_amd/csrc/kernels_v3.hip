@@ -936,7 +936,7 @@ __device__ double ref_solve(const double2 (&e)[kRefMax][kRefOff], const double2 
 
 // Row l of signal s's refine buckets: the offset g0 + l t (rows 0, 2, 3,
 // ... of the Mansour pre-pass above the LDS size, whose row 1 is the
-// reference's g0 + 1; else ref_ms, written by k_v3_pre).
+// reference's g0 + 1; else ref_ms, written by the refine pre tasks).
 __device__ __forceinline__ const double2 *ref_row(const V3Launch &a, int s, int l)
 {
   return a.ms_pre ? a.ms_pre + ((int64_t)s * a.ms_loops + (l ? l + 1 : 0)) * a.W
@@ -1058,13 +1058,89 @@ __device__ int refine_bucket(const V3Launch &a, int s, int g0, int j, const int 
 
 __device__ void write_result(const V3Launch &a, int s, const int *keys, const double2 *vals,
                              int M, int flags, double2 *lds);
+__device__ __forceinline__ void pre_task(const V3Launch &a, const int s, const int task,
+                                         double2 *lds, const size_t bytes, double2 *scr,
+                                         const size_t scr_bytes);
 
-__device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
+// Claim word of a pre task (per block and task 0..3): free, being computed
+// by the workgroup that claimed it, done.
+constexpr int kClaimFree = 0, kClaimBusy = 1, kClaimDone = 2;
+// Polls before a waiting transform gives up on a busy task (status bit 16).
+// A task is only ever busy in a running workgroup, which finishes it, so
+// this is a guard against a defect, not a path of normal runs.
+constexpr int kClaimSpins = 1 << 20;
+
+// Marks a finished pre task done.  The workgroup barrier orders every
+// thread's stores before lane 0's device-scope release (the L2 written
+// back once), then the flag.  All threads; contains a barrier.
+__device__ __forceinline__ void publish_task(int *cl)
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(cl, kClaimDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The results of pre tasks t0..t1 of signal s, before the transform reads
+// them.  Lane 0 claims every task no workgroup has started; the block runs
+// those here in turn (with pre_task's LDS arguments; no one else reads
+// them, so they stay claimed, not published); lane 0 then waits for the
+// others with plain polls and acquires once (one cache invalidation).
+// False if a wait gave up.  All threads; contains barriers.
+__device__ __forceinline__ bool need_tasks(const V3Launch &a, int s, int *cl, int t0, int t1,
+                                           double2 *lds, size_t bytes, double2 *scr,
+                                           size_t scr_bytes, int *sh)
+{
+  if (threadIdx.x == 0) {
+    int mine = 0;
+    for (int t = t0; t <= t1; t++)
+      if (atomicCAS(cl + t, kClaimFree, kClaimBusy) == kClaimFree) mine |= 1 << t;
+    *sh = mine;
+  }
+  __syncthreads();
+  const int mine = *sh;
+  __syncthreads();
+  for (int t = t0; t <= t1; t++)
+    if ((mine >> t) & 1) {
+      pre_task(a, s, t, lds, bytes, scr, scr_bytes);
+      __syncthreads();
+    }
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    bool others = false;
+    for (int t = t0; t <= t1; t++) {
+      if ((mine >> t) & 1) continue;
+      others = true;
+      int spins = 0;
+      while (__hip_atomic_load(cl + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kClaimDone)
+        if (++spins >= kClaimSpins) {
+          ok = 0;
+          break;
+        } else {
+          __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    if (others) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *sh = ok;
+  }
+  __syncthreads();
+  const bool ok = *sh;
+  __syncthreads();  // sh is reused
+  return ok;
+}
+
+__device__ __forceinline__ void v3_signal(const V3Launch &a, const int s, int *claims)
 {
   extern __shared__ double2 lds[];
   __shared__ int sh[16];
   __shared__ int s_rank[kV3Block];
-  __shared__ int s_size, s_flag, s_cnt[3], s_cm;
+  __shared__ int s_size, s_flag, s_cnt[3], s_cm, s_claim;
+  // the Mansour buckets (pre tasks 0, 1) first: a task run here uses the
+  // LDS from its start
+  // (and Gauss-1, task 2, which is done by then too: one wait for all three)
+  bool waits_ok = need_tasks(a, s, claims, 0, 2, lds, a.lds_bytes, nullptr, 0, &s_claim);
   Ctx c;
   c.a = &a;
   c.s = s;
@@ -1116,7 +1192,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   __threadfence_block();
   __syncthreads();
 
-  // ---- Mansour filter (cf3 96-127): buckets from k_v3_pre
+  // ---- Mansour filter (cf3 96-127): buckets from pre tasks 0, 1
   int F = est_mansour(c, s_rank, sh);
   if (!map_add_found(c, F, sh, &s_size)) {
     if (threadIdx.x == 0) s_flag |= 2;
@@ -1141,7 +1217,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
     return;
   }
 
-  // ---- Gauss filter 1 (cf3 130-212): buckets from k_v3_pre
+  // ---- Gauss filter 1 (cf3 130-212): buckets from pre task 2
   {
     const double2 *gp = a.gpre + (int64_t)s * (2 * a.B1 + 2 * a.B2);
     for (int r = threadIdx.x; r < a.B1; r += blockDim.x) {
@@ -1156,7 +1232,10 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
   __syncthreads();
   subtract(c, c.ef, c.ev, F, 1 | 4);
 
-  // ---- Gauss filter 2 (cf3 215-297): buckets from k_v3_pre
+  // ---- Gauss filter 2 (cf3 215-297): buckets from pre task 3 (run here:
+  // in its own bucket arrays' LDS, phases and samples in the staging LDS)
+  waits_ok &= need_tasks(a, s, claims, 3, 3, c.GP0, 2 * (size_t)a.B2 * sizeof(double2), c.stage,
+                         c.room, &s_claim);
   {
     const double2 *gp = a.gpre + (int64_t)s * (2 * a.B1 + 2 * a.B2) + 2 * a.B1;
     for (int r = threadIdx.x; r < a.B2; r += blockDim.x) {
@@ -1233,6 +1312,7 @@ __device__ __forceinline__ void v3_signal(const V3Launch &a, const int s)
     }
   }
   if (pass >= a.max_passes && threadIdx.x == 0) s_flag |= 4;
+  if (!waits_ok && threadIdx.x == 0) s_flag |= 16;
   __syncthreads();
   if (a.refine) {  // the refine kernels refine the map and write the result
     if (threadIdx.x == 0) {
@@ -1364,17 +1444,15 @@ __device__ __forceinline__ void gather_strided(double2 *lds, const double2 *x, u
 //             (cf3 66-87), the permuted fold and the FFTs;
 //   t = 4 + l (ntask = 4 + kRefOff when a group refines): the refine form's
 //             buckets at offset g0 + l ref_t.
-// Same operations in the same order as when k_v3_multi computed them, so
-// the buckets are the same bits.  The Gauss buckets go to gpre
-// (GS0 | GS1 | GP0 | GP1 per signal).
-__global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict__ groups,
-                                                     const int2 *__restrict__ blocks, int ntask)
+// Same operations in the same order whichever workgroup runs them, so the
+// buckets are the same bits.  The Gauss buckets go to gpre (GS0 | GS1 |
+// GP0 | GP1 per signal).  lds: the LDS the task may use (bytes of it;
+// task 3 puts its two bucket arrays there and its phases and samples in
+// scr, scr_bytes of LDS).
+__device__ __forceinline__ void pre_task(const V3Launch &a, const int s, const int task,
+                                         double2 *lds, const size_t bytes, double2 *scr,
+                                         const size_t scr_bytes)
 {
-  extern __shared__ double2 lds[];
-  const int2 bb = blocks[blockIdx.x / (unsigned)ntask];
-  const int task = (int)(blockIdx.x % (unsigned)ntask);
-  const V3Launch &a = groups[bb.x];
-  const int s = bb.y;
   const int n = a.n, W = a.W;
   const unsigned mask = (unsigned)n - 1;
   const int *prm = a.params + (int64_t)s * 6;
@@ -1451,17 +1529,17 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
   // by repeated complex multiplication; one lane replays that chain so the
   // samples carry the reference's rounding (into LDS past the bucket arrays
   // when it fits: a lone lane's global stores are several times slower)
+  // (the bucket arrays at lds, the phases and samples in scr)
   const int B = a.B2, chunks = a.w2 / B;
   double2 *GP0 = lds, *GP1 = lds + B;
-  const bool ph_lds = (2 * (size_t)B + a.w2 + 1) * sizeof(double2) <= a.lds_bytes;
-  double2 *const ph_s = lds + 2 * B;                     // LDS (typed stores)
+  const bool ph_lds = ((size_t)a.w2 + 1) * sizeof(double2) <= scr_bytes;
+  double2 *const ph_s = scr;                             // LDS (typed stores)
   double2 *const ph_g = a.phase + (int64_t)s * a.Pcap;  // HBM otherwise
   // while lane 0 runs the chain, waves 1.. gather the permuted samples
   // x[(gG + i ai) mod n], i <= w2 (cf3 78-85; gG, ai >= 0 and n a power of
   // two, so the mod is a mask) into LDS past the phases, where they fit
-  const bool staged =
-      (2 * (size_t)B + 2 * ((size_t)a.w2 + 1)) * sizeof(double2) <= a.lds_bytes;
-  double2 *xs = lds + 2 * B + a.w2 + 1;  // (an LDS address either way)
+  const bool staged = 2 * ((size_t)a.w2 + 1) * sizeof(double2) <= scr_bytes;
+  double2 *xs = scr + a.w2 + 1;  // (an LDS address either way)
   if (staged && threadIdx.x >= 64) {
     const int T1 = blockDim.x - 64;
     for (int i0 = threadIdx.x - 64; i0 <= a.w2; i0 += kPreAhead * T1) {
@@ -1522,24 +1600,56 @@ __global__ void __launch_bounds__(kV3Block) k_v3_pre(const V3Launch *__restrict_
   }
 }
 
-// Signals of several plans (different n, k) in one launch: block b runs
-// signal blocks[b].y of plan group blocks[b].x.
-__global__ void __launch_bounds__(kV3Block) k_v3_multi(const V3Launch *__restrict__ groups,
-                                                       const int2 *__restrict__ blocks)
+// The whole batch in one launch, signals of several plans (different n, k)
+// alike; blocks[j] = (plan group, signal).  Workgroups, in dispatch order:
+//   4 j + t (t < 4)    pre task t of block j, if no one has claimed it;
+//   4 nb + j           the signal's transform (v3_signal), which takes each
+//                      pre task's result once done, or claims and runs a
+//                      task no workgroup has started;
+//   5 nb + nref j + l  refine task 4 + l of block j (nref = kRefOff when a
+//                      group refines, else 0), read only by later kernels.
+// The transforms start while the Gauss-2 phase chains still run.
+__global__ void __launch_bounds__(kV3Block) k_v3_run(const V3Launch *__restrict__ groups,
+                                                     const int2 *__restrict__ blocks, int nb,
+                                                     int nref, int *__restrict__ claims)
 {
   // restrict: the descriptors are read-only here, so their fields need not
   // be reloaded after every store inside the signal's loops
-  const int2 b = blocks[blockIdx.x];
-  v3_signal(groups[b.x], b.y);
+  extern __shared__ double2 lds[];
+  __shared__ int s_go;
+  int w = (int)blockIdx.x;
+  if (w < 4 * nb) {
+    const int j = w >> 2, task = w & 3;
+    const int2 bb = blocks[j];
+    const V3Launch &a = groups[bb.x];
+    int *cl = claims + w;
+    if (threadIdx.x == 0) s_go = atomicCAS(cl, kClaimFree, kClaimBusy) == kClaimFree;
+    __syncthreads();
+    if (!s_go) return;
+    const size_t gp = 2 * (size_t)a.B2 * sizeof(double2);
+    pre_task(a, bb.y, task, lds, a.lds_bytes, lds + 2 * a.B2,
+             a.lds_bytes > gp ? a.lds_bytes - gp : 0);
+    publish_task(cl);
+    return;
+  }
+  w -= 4 * nb;
+  if (w < nb) {
+    const int2 bb = blocks[w];
+    v3_signal(groups[bb.x], bb.y, claims + 4 * w);
+    return;
+  }
+  w -= nb;
+  const int2 bb = blocks[w / nref];
+  pre_task(groups[bb.x], bb.y, 4 + w % nref, lds, groups[bb.x].lds_bytes, nullptr, 0);
 }
 
-// The refine form after k_v3_multi, in three kernels over the same
+// The refine form after k_v3_run, in three kernels over the same
 // (group, signal) blocks: k_v3_refine lists the located entries of every
 // residue class (linked lists through head, W entries in cf, and next, M
 // entries in remi); k_v3_refine_solve solves every bucket, one thread each,
 // nsub workgroups per signal; k_v3_refine_write sorts and writes the
 // result.  Groups without refine, and signals with the reference's early
-// return, wrote their result in k_v3_multi and are skipped.
+// return, wrote their result in k_v3_run and are skipped.
 __global__ void __launch_bounds__(kV3Block) k_v3_refine(const V3Launch *__restrict__ groups,
                                                         const int2 *__restrict__ blocks)
 {
@@ -1626,7 +1736,7 @@ static void allow_v3_lds(const void *kernel, size_t bytes)
 }
 
 int v3_multi(const V3Launch *h_groups, int G, const V3Launch *d_groups, const int2 *d_blocks,
-             int nblocks, hipStream_t st)
+             int nblocks, int *d_claims, hipStream_t st)
 {
   size_t lds = 0;
   for (int g = 0; g < G; g++) {
@@ -1640,24 +1750,17 @@ int v3_multi(const V3Launch *h_groups, int G, const V3Launch *d_groups, const in
       refine = true;
       nsub = std::max(nsub, (h_groups[g].W + kV3Block - 1) / kV3Block);
     }
-  const int ntask = refine ? 4 + kRefOff : 4;
-  allow_v3_lds(reinterpret_cast<const void *>(k_v3_pre), lds);
-  hipLaunchKernelGGL(k_v3_pre, dim3(ntask * nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks,
-                     ntask);
+  const int nref = refine ? kRefOff : 0;
+  allow_v3_lds(reinterpret_cast<const void *>(k_v3_run), lds);
+  hipLaunchKernelGGL(k_v3_run, dim3((5 + nref) * nblocks), dim3(kV3Block), lds, st, d_groups,
+                     d_blocks, nblocks, nref, d_claims);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    set_error(std::string("v3 pre: ") + hipGetErrorString(e));
-    return -1;
-  }
-  allow_v3_lds(reinterpret_cast<const void *>(k_v3_multi), lds);
-  hipLaunchKernelGGL(k_v3_multi, dim3(nblocks), dim3(kV3Block), lds, st, d_groups, d_blocks);
-  e = hipGetLastError();
-  if (e != hipSuccess) {
-    set_error(std::string("v3 multi: ") + hipGetErrorString(e));
+    set_error(std::string("v3 run: ") + hipGetErrorString(e));
     return -1;
   }
   if (refine) {
-    // groups without refine write their result in k_v3_multi; their blocks
+    // groups without refine write their result in k_v3_run; their blocks
     // return at once here
     hipLaunchKernelGGL(k_v3_refine, dim3(nblocks), dim3(kV3Block), 0, st, d_groups, d_blocks);
     hipLaunchKernelGGL(k_v3_refine_solve, dim3(nblocks * nsub), dim3(kV3Block), 0, st, d_groups,
