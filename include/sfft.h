@@ -224,10 +224,12 @@ int sfft_derive_params(int n, int k, sfft_version version, int *p10);
  * the reference's early return (src/computefourier-3.0.cc:859-860, empty
  * result), bit 1 result map overflow (the call returns -1 after writing
  * every signal's count and result, this signal's truncated to the map's
- * capacity), bit 2 the
- * estimate/subtract loop stopped at its 3*4096-pass cap instead of at
- * repeated bucket counts, bit 3 the refine form kept the reference's values
- * in a bucket it could not solve.  Copies up to num flags; returns the number of
+ * capacity), bit 2 the estimate/subtract loop stopped at its 3*4096-pass
+ * cap instead of at repeated bucket counts, bit 3 the refine form kept the
+ * reference's values in a bucket it could not solve, bit 4 the transform
+ * gave up waiting (about a second) for a bucket task another workgroup had
+ * started -- a library defect, never seen in normal runs; the call then
+ * returns -1 as for bit 1, and that signal's result is not valid.  Copies up to num flags; returns the number of
  * signals recorded (0 for v1/v2 plans), or -1 for a bad plan. */
 int sfft_plan_last_status(const sfft_plan *plan, int *flags, int num);
 
@@ -271,6 +273,14 @@ int64_t sfft_debug_comb_votes(const sfft_plan *plan, int signal, int32_t *hits, 
  * sfft_plan_output_capacity), to exercise the overflow handling; 0 keeps
  * the plan's capacity.  Returns 0, or -1 for a bad or non-v3 plan. */
 int sfft_plan_debug_v3(sfft_plan *plan, int serial_subtractions, int map_limit);
+
+/* Diagnostic of a v3 plan's later execs: enabled != 0 makes each signal's
+ * transform workgroup run its four pre tasks (Mansour and Gauss buckets)
+ * itself instead of taking them from the workgroups that normally run
+ * them ahead -- the path a transform takes when it finds a task not yet
+ * started.  Results are bit-identical either way (the tests compare them).
+ * Returns 0, or -1 for a bad or non-v3 plan. */
+int sfft_plan_debug_v3_inline_pre(sfft_plan *plan, int enabled);
 
 /* The plan's filters (which: 0 = location / first Gauss filter, 1 =
  * estimation / second Gauss filter): its w taps into time (capacity

@@ -23,6 +23,7 @@ EXPORTS = [
     "sfft_exec_many_plans", "sfft_plan_enable_graphs", "sfft_generate_input",
     "sfft_exec_sparse_noalloc", "sfft_plan_debug_comb_votes", "sfft_debug_comb_votes",
     "sfft_plan_debug_v3",
+    "sfft_plan_debug_v3_inline_pre",
 ]
 
 # Every symbol include/sfft_server.h declares.
@@ -90,6 +91,7 @@ def load():
     L.sfft_exec_sparse_noalloc.argtypes = [PP, P, C.POINTER(SfftOutput)]
     L.sfft_plan_debug_comb_votes.argtypes = [PP, C.c_int]
     L.sfft_plan_debug_v3.argtypes = [PP, C.c_int, C.c_int]
+    L.sfft_plan_debug_v3_inline_pre.argtypes = [PP, C.c_int]
     L.sfft_debug_comb_votes.argtypes = [PP, C.c_int, P, C.c_int64]
     L.sfft_debug_comb_votes.restype = C.c_int64
     L.sfft_plan_set_precision.argtypes = [PP, C.c_int]

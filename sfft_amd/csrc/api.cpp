@@ -672,6 +672,19 @@ int sfft_plan_debug_v3(sfft_plan *plan, int serial_subtractions, int map_limit)
   return 0;
 }
 
+int sfft_plan_debug_v3_inline_pre(sfft_plan *plan, int enabled)
+{
+  Plan *P = get(plan);
+  if (!P) return -1;
+  if (P->version != SFFT_VERSION_3) {
+    set_error("the v3 diagnostics apply to v3 plans only");
+    return -1;
+  }
+  auto lk = hold(P);
+  P->debug_v3_inline_pre = enabled != 0;
+  return 0;
+}
+
 int64_t sfft_debug_comb_votes(const sfft_plan *plan, int signal, int32_t *hits, int64_t cap)
 {
   Plan *P = get(plan);

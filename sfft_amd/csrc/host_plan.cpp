@@ -676,6 +676,7 @@ static int v3_prepare(Plan *P, BatchIO &io, V3Pending &pd, V3Stage &sg)
   a.ms_loops = ms_loops;
   a.refine = refine ? 1 : 0;
   a.serial_all = P->debug_v3_serial ? 1 : 0;
+  a.inline_pre = P->debug_v3_inline_pre ? 1 : 0;
   a.map_limit = P->debug_v3_map_limit > 0 ? std::min(P->debug_v3_map_limit, P->v3_Mcap)
                                           : P->v3_Mcap;
   a.ref_ms = refine && !big_ms ? reinterpret_cast<double2 *>(w) : nullptr;
@@ -719,11 +720,16 @@ static int v3_finish(Plan *P, BatchIO &io, const V3Pending &pd)
   P->last_status.insert(P->last_status.end(), pd.h_status, pd.h_status + S);
   if (io.counts)
     for (int s = 0; s < S; s++) io.counts[s] = pd.h_counts[s];
-  for (int s = 0; s < S; s++)
+  for (int s = 0; s < S; s++) {
     if (pd.h_status[s] & 2) {
       set_error("v3 result map overflow (more located frequencies than the plan holds)");
       return 1;
     }
+    if (pd.h_status[s] & 16) {
+      set_error("v3: a transform gave up waiting for a bucket task (library defect)");
+      return 1;
+    }
+  }
   return 0;
 }
 

@@ -195,6 +195,7 @@ struct Plan {
   bool debug_votes = false;  // v2 FP64: count the comb votes (diagnostic)
   bool debug_v3_serial = false;  // v3: every subtraction pass by one thread per array
   int debug_v3_map_limit = 0;     // v3: map capacity below Mcap (0: Mcap)
+  bool debug_v3_inline_pre = false;  // v3: transforms run their own pre tasks
   int64_t votes_stride = 0;  // int64 per signal in ws->votes of the last batch
   Rng rng;
   hipStream_t stream = nullptr;
@@ -402,6 +403,7 @@ struct V3Launch {
   int ref_t, ref_tinv;
   int serial_all;            // every subtraction pass serial (diagnostic)
   int map_limit;             // map entries allowed (Mcap, or lower for diagnostics)
+  int inline_pre;            // pre-task workgroups leave their tasks (diagnostic)
   double2 *ref_ms;
   int *map_keys;             // Mcap (insertion order)
   double2 *map_vals;         // Mcap

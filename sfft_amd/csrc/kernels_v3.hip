@@ -1623,6 +1623,7 @@ __global__ void __launch_bounds__(kV3Block) k_v3_run(const V3Launch *__restrict_
     const int2 bb = blocks[j];
     const V3Launch &a = groups[bb.x];
     int *cl = claims + w;
+    if (a.inline_pre) return;  // diagnostic: the transform runs the task
     if (threadIdx.x == 0) s_go = atomicCAS(cl, kClaimFree, kClaimBusy) == kClaimFree;
     __syncthreads();
     if (!s_go) return;

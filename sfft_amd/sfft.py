@@ -265,7 +265,8 @@ class sfft:
     def last_status(self):
         """v3 flags per signal of the last exec call: 1 early return,
         2 map overflow, 4 loop stopped at the pass cap, 8 the refine form
-        kept the reference's values in some bucket."""
+        kept the reference's values in some bucket, 16 a wait for a bucket
+        task gave up (a defect; the result is not valid)."""
         m = self._L.sfft_plan_last_status(self.sfft_plan, None, 0)
         if m < 0:
             raise SfftError(_lib.last_error())
@@ -300,6 +301,11 @@ class sfft:
         map of located frequencies capped at map_limit entries."""
         if self._L.sfft_plan_debug_v3(self.sfft_plan, int(bool(serial_subtractions)),
                                       int(map_limit)) != 0:
+            raise SfftError(_lib.last_error())
+
+    def debug_v3_inline_pre(self, enabled=True):
+        """v3 plans (diagnostic): each transform runs its own pre tasks."""
+        if self._L.sfft_plan_debug_v3_inline_pre(self.sfft_plan, int(bool(enabled))) != 0:
             raise SfftError(_lib.last_error())
 
     def debug_comb_votes(self, enabled=True):

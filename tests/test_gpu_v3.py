@@ -280,6 +280,37 @@ def test_v3_owner_apply_equals_serial(torch_cuda, n, k):
         assert torch.equal(va[i, :c], vb[i, :c])
 
 
+@pytest.mark.parametrize("n,k,precision", [(1 << 24, 1000, "fp64"), (1 << 22, 2000, "fp64"),
+                                         (1 << 20, 3000, "fp64"), (1 << 20, 200, "fp32_refine")])
+def test_v3_transforms_running_pre_tasks_equal(torch_cuda, n, k, precision):
+    """Each signal's four pre tasks (Mansour loops, Gauss-1, Gauss-2 with its
+    phase chain) normally run in workgroups ahead of the transforms; a
+    transform that finds a task not started claims and runs it itself, in
+    its own LDS (Gauss-2 beside its live Gauss-1 buckets).  With
+    sfft_plan_debug_v3_inline_pre every transform takes that path: outputs
+    bit-identical.  Covers W = 8192 in LDS, W = 16384 from the Mansour
+    pre-pass, k = 3000 (phases in HBM when run by the transform) and the
+    refine form (its refine tasks still run ahead)."""
+    torch = torch_cuda
+    print(f"case v3 inline pre tasks n={n} k={k} {precision}", flush=True)
+    S = 3
+    xd = [torch.from_numpy(signal_np(n, k, seed=9300 + 5 * i + k)[0]).cuda() for i in range(S)]
+    a = sfft_amd.sfft(n, k, 3, precision=precision, seed=(47, 53))
+    b = sfft_amd.sfft(n, k, 3, precision=precision, seed=(47, 53))
+    b.debug_v3_inline_pre(True)
+    fa, va = a.make_sparse_outputs(S)
+    fb, vb = b.make_sparse_outputs(S)
+    ca = a.execute_many_sparse(xd, fa, va)
+    cb = b.execute_many_sparse(xd, fb, vb)
+    torch.cuda.synchronize()
+    assert np.array_equal(ca, cb) and int(ca.min()) > 0
+    assert not any(int(f) & 16 for f in b.last_status())
+    for i in range(S):
+        c = int(ca[i])
+        assert torch.equal(fa[i, :c], fb[i, :c])
+        assert torch.equal(va[i, :c], vb[i, :c])
+
+
 def test_v3_map_overflow_writes_every_result(torch_cuda):
     """A v3 map overflow, forced from the host (sfft_plan_debug_v3 caps the
     map at 12 entries), fails the call but writes every signal's count and
