@@ -416,6 +416,35 @@ __device__ void mansour_removal(const Ctx &c, int key, double2 value, int &h, do
 // Subtraction values of cnt (key, value) pairs, then their application in
 // pair order.  which: bit 0 Gauss-1, bit 1 Gauss-2, bit 2 Mansour.
 // Slots per pair in rem: [0..5] G1, [6..11] G2, [12..13] Mansour.
+// The removal values of cnt (key, value) pairs for the filters in which
+// (bit 0 Gauss-1, 1 Gauss-2, 2 Mansour) into r (14 per pair) and ri (3
+// bucket indices per pair).  One work item per (filter, pair); each
+// filter's pairs start on a wave boundary, so a wave runs one filter's
+// code and the filters' sincos and window loads proceed in parallel.
+__device__ __forceinline__ void removals(const Ctx &c, const int *keys, const double2 *vals,
+                                         int cnt, int which, double2 *r0, int *ri0)
+{
+  const V3Launch &a = *c.a;
+  int fs[3], nf = 0;
+  for (int f = 0; f < 3; f++)
+    if ((which >> f) & 1) fs[nf++] = f;
+  const int stride = (cnt + 63) & ~63;
+  for (int t = threadIdx.x; t < nf * stride; t += blockDim.x) {
+    const int q = t / stride, i = t - q * stride;
+    if (i >= cnt) continue;
+    const int f = fs[q];
+    const int key = keys[i];
+    const double2 v = vals[i];
+    double2 *r = r0 + (int64_t)i * 14;
+    int *ri = ri0 + (int64_t)i * 3;
+    if (f == 0) gauss_removal(c, key, v, c.B1, a.win1, a.win1_half, c.gG, 1, ri[0], r);
+    else if (f == 1)
+      gauss_removal(c, (timesmod(key, c.ai, c.n) + c.shift) % c.n, v, c.B2, a.win2,
+                    a.win2_half, wmul(c.aa, c.gG), 1, ri[1], r + 6);
+    else mansour_removal(c, key, v, ri[2], r + 12);
+  }
+}
+
 __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const double2 *vals,
                                          int cnt, int which)
 {
@@ -430,32 +459,7 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
     // (in HBM) is updated by its owner thread h mod kV3Block in pair order
     // -- every bucket sees the same subtractions in the same order.
     int *si = reinterpret_cast<int *>(sv + 14 * kSerialApply);
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-      const int key = keys[i];
-      const double2 v = vals[i];
-      double2 *r = sv + i * 14;
-      int *ri = si + i * 3;
-      // both filters' window values loaded together
-      const int key2 = (timesmod(key, c.ai, c.n) + c.shift) % c.n;
-      GaussIdx x1 = {}, x2 = {};
-      double2 F1[3], F2[3];
-      if (which & 1) x1 = gauss_index(c, key, c.B1);
-      if (which & 2) x2 = gauss_index(c, key2, c.B2);
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        if (which & 1) F1[q] = win_at(a.win1, a.win1_half, x1.dd[q], c.n);
-        if (which & 2) F2[q] = win_at(a.win2, a.win2_half, x2.dd[q], c.n);
-      }
-      if (which & 1) {
-        ri[0] = x1.h;
-        gauss_values(c, key, v, c.gG, 1, F1, r);
-      }
-      if (which & 2) {
-        ri[1] = x2.h;
-        gauss_values(c, key2, v, wmul(c.aa, c.gG), 1, F2, r + 6);
-      }
-      if (which & 4) mansour_removal(c, key, v, ri[2], r + 12);
-    }
+    removals(c, keys, vals, cnt, which, sv, si);
     __syncthreads();
     if (which & 1) gauss_apply_few(c.GS0, c.GS1, c.B1, si, 0, sv, 0, cnt);
     if (which & 2) gauss_apply_few(c.GP0, c.GP1, c.B2, si, 1, sv, 6, cnt);
@@ -470,17 +474,7 @@ __device__ __forceinline__ void subtract(const Ctx &c, const int *keys, const do
     __syncthreads();
     return;
   }
-  for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-    const int key = keys[i];
-    const double2 v = vals[i];
-    double2 *r = c.rem + (int64_t)i * 14;
-    int *ri = c.remi + (int64_t)i * 3;
-    if (which & 1) gauss_removal(c, key, v, c.B1, a.win1, a.win1_half, c.gG, 1, ri[0], r);
-    if (which & 2)
-      gauss_removal(c, (timesmod(key, c.ai, c.n) + c.shift) % c.n, v, c.B2, a.win2,
-                    a.win2_half, wmul(c.aa, c.gG), 1, ri[1], r + 6);
-    if (which & 4) mansour_removal(c, key, v, ri[2], r + 12);
-  }
+  removals(c, keys, vals, cnt, which, c.rem, c.remi);
   __threadfence_block();
   __syncthreads();
   // One thread per bucket, each taking the pairs in pair order: a bucket
