@@ -1641,8 +1641,9 @@ __global__ void __launch_bounds__(kV3Block) k_v3_run(const V3Launch *__restrict_
 // The refine form after k_v3_run, in three kernels over the same
 // (group, signal) blocks: k_v3_refine lists the located entries of every
 // residue class (linked lists through head, W entries in cf, and next, M
-// entries in remi); k_v3_refine_solve solves every bucket, one thread each,
-// nsub workgroups per signal; k_v3_refine_write sorts and writes the
+// entries in remi) and the non-empty classes; k_v3_refine_solve solves
+// those, one thread each (nsub workgroups per signal, the later ones
+// mostly idle); k_v3_refine_write sorts and writes the
 // result.  Groups without refine, and signals with the reference's early
 // return, wrote their result in k_v3_run and are skipped.
 __global__ void __launch_bounds__(kV3Block) k_v3_refine(const V3Launch *__restrict__ groups,
@@ -1655,11 +1656,22 @@ __global__ void __launch_bounds__(kV3Block) k_v3_refine(const V3Launch *__restri
   const int M = (int)a.count[s], W = a.W;
   const int *keys = a.map_keys + (int64_t)s * a.Mcap;
   int *head = a.cf + (int64_t)s * a.Fcap, *next = a.remi + (int64_t)s * a.Rcap * 3;
+  // the non-empty classes, for the solve (in cv, free after the loop):
+  // nz[0] their count, nz[1..] the classes in no particular order
+  int *nz = reinterpret_cast<int *>(a.cv + (int64_t)s * a.Fcap);
+  __shared__ int s_nz;
   for (int j = threadIdx.x; j < W; j += blockDim.x) head[j] = -1;
+  if (threadIdx.x == 0) s_nz = 0;
   __threadfence_block();
   __syncthreads();
-  for (int e = threadIdx.x; e < M; e += blockDim.x)
-    next[e] = atomicExch(&head[keys[e] & (W - 1)], e);
+  for (int e = threadIdx.x; e < M; e += blockDim.x) {
+    const int j = keys[e] & (W - 1);
+    const int old = atomicExch(&head[j], e);
+    next[e] = old;
+    if (old < 0) nz[1 + atomicAdd(&s_nz, 1)] = j;  // first entry of class j
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) nz[0] = s_nz;
 }
 
 __global__ void __launch_bounds__(kV3Block) k_v3_refine_solve(const V3Launch *__restrict__ groups,
@@ -1667,10 +1679,15 @@ __global__ void __launch_bounds__(kV3Block) k_v3_refine_solve(const V3Launch *__
                                                               int nsub)
 {
   const int2 bb = blocks[blockIdx.x / (unsigned)nsub];
-  const int j = (int)(blockIdx.x % (unsigned)nsub) * blockDim.x + threadIdx.x;
+  const int jj = (int)(blockIdx.x % (unsigned)nsub) * blockDim.x + threadIdx.x;
   const V3Launch &a = groups[bb.x];
   const int s = bb.y;
-  if (!a.refine || j >= a.W || (a.status[s] & 1)) return;  // no barriers here
+  if (!a.refine || (a.status[s] & 1)) return;  // no barriers here
+  // one thread per non-empty class (listed by k_v3_refine): every lane of
+  // a wave has a bucket to solve
+  const int *nz = reinterpret_cast<const int *>(a.cv + (int64_t)s * a.Fcap);
+  if (jj >= nz[0]) return;
+  const int j = nz[1 + jj];
   const int g0 = a.params[(int64_t)s * 6 + 4];
   if (refine_bucket(a, s, g0, j, a.cf + (int64_t)s * a.Fcap, a.remi + (int64_t)s * a.Rcap * 3,
                     a.map_keys + (int64_t)s * a.Mcap, a.map_vals + (int64_t)s * a.Mcap))
