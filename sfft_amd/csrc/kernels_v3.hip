@@ -12,6 +12,7 @@
 // result map is iterated in insertion order (the reference's
 // tr1::unordered_map order is unreproducible; see the oracle).
 #include <algorithm>
+#include <climits>
 
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -885,49 +886,6 @@ __device__ __forceinline__ long long ref_snap(double ang, int j, int n, int W, i
   return j + (long long)W * ((qt * tinv) & np1);
 }
 
-// Least squares over kRefOff equations: y[l] ~ sum_m X[m] e[m][l], p <=
-// kRefMax unknowns, by modified Gram-Schmidt QR of the columns e[m] (the
-// columns are samples of unit exponentials; QR keeps their conditioning
-// instead of squaring it).  Returns the squared residual (1e300 when the
-// columns are dependent).
-__device__ double ref_solve(const double2 (&e)[kRefMax][kRefOff], const double2 (&y)[kRefOff],
-                           int p, double2 (&X)[kRefMax])
-{
-  double2 Q[kRefMax][kRefOff], R[kRefMax][kRefMax], z[kRefMax];
-  for (int m = 0; m < p; m++) {
-    for (int l = 0; l < kRefOff; l++) Q[m][l] = e[m][l];
-    for (int q = 0; q < m; q++) {
-      double2 d = make_double2(0, 0);
-      for (int l = 0; l < kRefOff; l++) d = cadd(d, cconjmul(Q[q][l], Q[m][l]));
-      R[q][m] = d;
-      for (int l = 0; l < kRefOff; l++) Q[m][l] = csub(Q[m][l], cmul(d, Q[q][l]));
-    }
-    double nn = 0;
-    for (int l = 0; l < kRefOff; l++) nn += Q[m][l].x * Q[m][l].x + Q[m][l].y * Q[m][l].y;
-    if (!(nn > 1e-24)) return 1e300;
-    const double inv = 1.0 / sqrt(nn);
-    R[m][m] = make_double2(sqrt(nn), 0);
-    for (int l = 0; l < kRefOff; l++) Q[m][l] = make_double2(Q[m][l].x * inv, Q[m][l].y * inv);
-  }
-  for (int m = 0; m < p; m++) {
-    double2 d = make_double2(0, 0);
-    for (int l = 0; l < kRefOff; l++) d = cadd(d, cconjmul(Q[m][l], y[l]));
-    z[m] = d;
-  }
-  for (int m = p - 1; m >= 0; m--) {
-    double2 acc = z[m];
-    for (int q = m + 1; q < p; q++) acc = csub(acc, cmul(R[m][q], X[q]));
-    X[m] = make_double2(acc.x / R[m][m].x, acc.y / R[m][m].x);
-  }
-  double res = 0;
-  for (int l = 0; l < kRefOff; l++) {
-    double2 r = y[l];
-    for (int m = 0; m < p; m++) r = csub(r, cmul(X[m], e[m][l]));
-    res += r.x * r.x + r.y * r.y;
-  }
-  return res;
-}
-
 // Row l of signal s's refine buckets: the offset g0 + l t (rows 0, 2, 3,
 // ... of the Mansour pre-pass above the LDS size, whose row 1 is the
 // reference's g0 + 1; else ref_ms, written by the refine pre tasks).
@@ -937,90 +895,204 @@ __device__ __forceinline__ const double2 *ref_row(const V3Launch &a, int s, int 
                   : a.ref_ms + ((int64_t)s * kRefOff + l) * a.W;
 }
 
-// The refine solve of bucket j of signal s (one thread): members from the
-// lists head/next (entries of keys/vals), values written to vals.  Returns
-// 1 when the bucket keeps the reference's values.
-__device__ int refine_bucket(const V3Launch &a, int s, int g0, int j, const int *head,
-                             const int *next, const int *keys, double2 *vals)
+// ---------------------------------------------------------------- refine solve
+// Each non-empty residue class is solved by a group of kRefOff lanes, lane
+// l holding offset l: its bucket value, its entries of the columns, its row
+// of R.  Every sum over the offsets is taken in offset order from zero,
+// term by term, by every lane of the group (gsum*), so every lane holds
+// the same sums -- those of one thread looping over the offsets -- and the
+// group's control flow stays uniform.
+__device__ __forceinline__ double gsum(double v, int lim = kRefOff)
+{
+  double acc = 0;
+#pragma unroll
+  for (int l = 0; l < kRefOff; l++) {
+    const double t = __shfl(v, l, kRefOff);
+    if (l < lim) acc += t;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double2 gsum2(double2 v, int lim = kRefOff)
+{
+  double2 acc = make_double2(0, 0);
+#pragma unroll
+  for (int l = 0; l < kRefOff; l++) {
+    const double tx = __shfl(v.x, l, kRefOff), ty = __shfl(v.y, l, kRefOff);
+    if (l < lim) acc = cadd(acc, make_double2(tx, ty));
+  }
+  return acc;
+}
+
+// 0 - t_0 - t_1 - ... over lanes l < lim (a subtracted sum)
+__device__ __forceinline__ double2 gsub2(double2 v, int lim)
+{
+  double2 acc = make_double2(0, 0);
+#pragma unroll
+  for (int l = 0; l < kRefOff; l++) {
+    const double tx = __shfl(v.x, l, kRefOff), ty = __shfl(v.y, l, kRefOff);
+    if (l < lim) acc = csub(acc, make_double2(tx, ty));
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double2 gshfl2(double2 v, int src)
+{
+  return make_double2(__shfl(v.x, src, kRefOff), __shfl(v.y, src, kRefOff));
+}
+
+// Least squares over the kRefOff equations: y_l ~ sum_m X[m] e[m]_l, pc <=
+// kRefMax unknowns, by modified Gram-Schmidt QR of the columns (samples of
+// unit exponentials: QR keeps their conditioning instead of squaring it).
+// e[m] is this lane's entry of column m, y its bucket value.  Returns the
+// squared residual (1e300 when the columns are dependent); X on every lane.
+__device__ __forceinline__ double ref_solve_lanes(const double2 (&e)[kRefMax], double2 y,
+                                                  int pc, double2 (&X)[kRefMax], int lane)
+{
+  double2 Q[kRefMax], Rrow[kRefMax], z[kRefMax];  // Rrow: row `lane` of R
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++) {
+    Rrow[m] = make_double2(0, 0);
+    Q[m] = make_double2(0, 0);
+    z[m] = make_double2(0, 0);
+  }
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++) {
+    if (m >= pc) break;
+    Q[m] = e[m];
+#pragma unroll
+    for (int q = 0; q < m; q++) {
+      const double2 d = gsum2(cconjmul(Q[q], Q[m]));
+      if (lane == q) Rrow[m] = d;
+      Q[m] = csub(Q[m], cmul(d, Q[q]));
+    }
+    const double nn = gsum(Q[m].x * Q[m].x + Q[m].y * Q[m].y);
+    if (!(nn > 1e-24)) return 1e300;
+    const double inv = 1.0 / sqrt(nn);
+    if (lane == m) Rrow[m] = make_double2(sqrt(nn), 0);
+    Q[m] = make_double2(Q[m].x * inv, Q[m].y * inv);
+  }
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++)
+    if (m < pc) z[m] = gsum2(cconjmul(Q[m], y));
+  // back substitution: row m lives on lane m, X[m] is then broadcast
+#pragma unroll
+  for (int m = kRefMax - 1; m >= 0; m--) {
+    if (m >= pc) continue;
+    double2 acc = z[m];
+#pragma unroll
+    for (int q = m + 1; q < kRefMax; q++)
+      if (q < pc) acc = csub(acc, cmul(Rrow[q], X[q]));
+    const double2 xm = make_double2(acc.x / Rrow[m].x, acc.y / Rrow[m].x);
+    X[m] = gshfl2(xm, m);
+  }
+  double2 r = y;
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++)
+    if (m < pc) r = csub(r, cmul(X[m], e[m]));
+  return gsum(r.x * r.x + r.y * r.y);
+}
+
+// The refine solve of class j of signal s by a lane group (lane = this
+// lane's offset): members from the lists head/next (entries of keys/vals),
+// values written to vals.  Returns 1 when the bucket keeps the reference's
+// values.
+__device__ __forceinline__ int refine_class(const V3Launch &a, int s, int g0, int j,
+                                            const int *head, const int *next, const int *keys,
+                                            double2 *vals, int lane)
 {
   const int n = a.n, W = a.W, t = a.ref_t, tinv = a.ref_tinv;
   const unsigned mask = (unsigned)n - 1;
   int mem[kRefMax];
+  long long fk[kRefMax];
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++) mem[m] = 0;  // entry 0 exists (p >= 1 below)
   int p = 0;
+  bool over = false;
   for (int e = head[j]; e >= 0; e = next[e]) {
     if (p == kRefMax) {
-      p = kRefMax + 1;
+      over = true;
       break;
     }
-    mem[p++] = e;
+#pragma unroll
+    for (int m = 0; m < kRefMax; m++)
+      if (m == p) mem[m] = e;
+    p++;
   }
   if (p == 0) return 0;
-  if (p > kRefMax || p >= kRefOff) return 1;  // no spare equation: keep the reference values
-  // members by frequency (the list order depends on the atomics)
-  for (int u = 1; u < p; u++)
-    for (int v = u; v > 0 && keys[mem[v]] < keys[mem[v - 1]]; v--) {
-      const int tmp = mem[v];
-      mem[v] = mem[v - 1];
-      mem[v - 1] = tmp;
+  if (over || p >= kRefOff) return 1;  // no spare equation: keep the reference values
+  // members by frequency (distinct keys: the order insertion sorting gives)
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++) fk[m] = m < p ? (long long)keys[mem[m]] : LLONG_MAX;
+#pragma unroll
+  for (int round = 0; round < kRefMax; round++)
+#pragma unroll
+    for (int i = round & 1; i + 1 < kRefMax; i += 2)
+      if (fk[i] > fk[i + 1]) {
+        const long long tf = fk[i];
+        fk[i] = fk[i + 1];
+        fk[i + 1] = tf;
+        const int tm = mem[i];
+        mem[i] = mem[i + 1];
+        mem[i + 1] = tm;
+      }
+  const double2 mv = ref_row(a, s, lane)[j];
+  const double2 y = make_double2(mv.x / W, mv.y / W);
+  const double scale = gsum(y.x * y.x + y.y * y.y);
+  double2 e[kRefMax], w[kRefMax], X[kRefMax];
+#pragma unroll
+  for (int m = 0; m < kRefMax; m++) {
+    e[m] = make_double2(0, 0);
+    w[m] = make_double2(0, 0);
+    X[m] = make_double2(0, 0);
+    if (m < p) {
+      e[m] = cis_mod((fk[m] * ((long long)g0 + (long long)lane * t)) & mask, n);
+      w[m] = cis_mod((fk[m] * t) & mask, n);
     }
-  double2 y[kRefOff], e[kRefMax][kRefOff], X[kRefMax], w[kRefMax];
-  double scale = 0;
-  for (int l = 0; l < kRefOff; l++) {
-    const double2 m = ref_row(a, s, l)[j];
-    y[l] = make_double2(m.x / W, m.y / W);
-    scale += y[l].x * y[l].x + y[l].y * y[l].y;
   }
-  auto column = [&](int m, long long f) {
-    for (int l = 0; l < kRefOff; l++)
-      e[m][l] = cis_mod((f * ((long long)g0 + (long long)l * t)) & mask, n);
-  };
-  for (int m = 0; m < p; m++) {
-    const long long f = keys[mem[m]];
-    column(m, f);
-    w[m] = cis_mod((f * t) & mask, n);
-  }
-  // residual at most 1e-10 of the bucket (or of 1e-4, below which a
-  // bucket holds rounding only)
   const double tol = 1e-20 * (scale > 1e-8 ? scale : 1e-8);
-  double res = ref_solve(e, y, p, X);
+  double res = ref_solve_lanes(e, y, p, X, lane);
   if (res > tol) {
     // annihilate the members' nodes: d_l <- d_{l+1} - w_m d_l, once per member
-    double2 d[kRefOff];
+    double2 d = y;
     int len = kRefOff;
-    for (int l = 0; l < kRefOff; l++) d[l] = y[l];
-    for (int m = 0; m < p; m++) {
-      for (int l = 0; l + 1 < len; l++) d[l] = csub(d[l + 1], cmul(w[m], d[l]));
+#pragma unroll
+    for (int m = 0; m < kRefMax; m++) {
+      if (m >= p) break;
+      const double2 dn = gshfl2(d, lane + 1 < kRefOff ? lane + 1 : lane);
+      if (lane + 1 < len) d = csub(dn, cmul(w[m], d));
       len--;
     }
     double2 X2[kRefMax];
     if (p + 1 <= kRefMax && len >= 2) {  // one unlocated tone: w_u = d_{l+1} / d_l
-      double2 num = make_double2(0, 0);
-      double den = 0;
-      for (int l = 0; l + 1 < len; l++) {
-        num = cadd(num, cconjmul(d[l], d[l + 1]));
-        den += d[l].x * d[l].x + d[l].y * d[l].y;
-      }
+      const double2 d1 = gshfl2(d, lane + 1 < kRefOff ? lane + 1 : lane);
+      const double2 num = gsum2(cconjmul(d, d1), len - 1);
+      const double den = gsum(d.x * d.x + d.y * d.y, len - 1);
       if (den > 0) {
-        column(p, ref_snap(atan2(num.y, num.x), j, n, W, t, tinv));
-        const double res2 = ref_solve(e, y, p + 1, X2);
+        const long long u = ref_snap(atan2(num.y, num.x), j, n, W, t, tinv);
+#pragma unroll
+        for (int m = 0; m < kRefMax; m++)
+          if (m == p) e[m] = cis_mod((u * ((long long)g0 + (long long)lane * t)) & mask, n);
+        const double res2 = ref_solve_lanes(e, y, p + 1, X2, lane);
         if (res2 <= tol) {
           res = res2;
-          for (int m = 0; m < p; m++) X[m] = X2[m];
+#pragma unroll
+          for (int m = 0; m < kRefMax; m++)
+            if (m < p) X[m] = X2[m];
         }
       }
     }
     if (res > tol && p + 2 <= kRefMax && len >= 4) {
       // two unlocated tones: least-squares predictor over l < len - 2,
       // normal equations of the columns (d_{l+1}, d_l) against -d_{l+2}
-      double g11 = 0, g22 = 0;
-      double2 g12 = make_double2(0, 0), b1 = make_double2(0, 0), b2 = make_double2(0, 0);
-      for (int l = 0; l + 2 < len; l++) {
-        g11 += d[l + 1].x * d[l + 1].x + d[l + 1].y * d[l + 1].y;
-        g22 += d[l].x * d[l].x + d[l].y * d[l].y;
-        g12 = cadd(g12, cconjmul(d[l + 1], d[l]));
-        b1 = csub(b1, cconjmul(d[l + 1], d[l + 2]));
-        b2 = csub(b2, cconjmul(d[l], d[l + 2]));
-      }
+      const int lim = len - 2;
+      const double2 d1 = gshfl2(d, lane + 1 < kRefOff ? lane + 1 : lane);
+      const double2 d2 = gshfl2(d, lane + 2 < kRefOff ? lane + 2 : lane);
+      const double g11 = gsum(d1.x * d1.x + d1.y * d1.y, lim);
+      const double g22 = gsum(d.x * d.x + d.y * d.y, lim);
+      const double2 g12 = gsum2(cconjmul(d1, d), lim);
+      const double2 b1 = gsub2(cconjmul(d1, d2), lim);
+      const double2 b2 = gsub2(cconjmul(d, d2), lim);
       const double det = g11 * g22 - (g12.x * g12.x + g12.y * g12.y);
       if (det > 0) {
         const double2 g21 = make_double2(g12.x, -g12.y);
@@ -1034,19 +1106,28 @@ __device__ int refine_bucket(const V3Launch &a, int s, int g0, int j, const int 
         const long long u1 = ref_snap(atan2(z1.y, z1.x), j, n, W, t, tinv);
         const long long u2 = ref_snap(atan2(z2.y, z2.x), j, n, W, t, tinv);
         if (u1 != u2) {
-          column(p, u1);
-          column(p + 1, u2);
-          const double res2 = ref_solve(e, y, p + 2, X2);
+#pragma unroll
+          for (int m = 0; m < kRefMax; m++) {
+            if (m == p) e[m] = cis_mod((u1 * ((long long)g0 + (long long)lane * t)) & mask, n);
+            if (m == p + 1) e[m] = cis_mod((u2 * ((long long)g0 + (long long)lane * t)) & mask, n);
+          }
+          const double res2 = ref_solve_lanes(e, y, p + 2, X2, lane);
           if (res2 <= tol) {
             res = res2;
-            for (int m = 0; m < p; m++) X[m] = X2[m];
+#pragma unroll
+            for (int m = 0; m < kRefMax; m++)
+              if (m < p) X[m] = X2[m];
           }
         }
       }
     }
   }
   if (res > tol) return 1;
-  for (int m = 0; m < p; m++) vals[mem[m]] = X[m];
+  if (lane == 0) {
+#pragma unroll
+    for (int m = 0; m < kRefMax; m++)
+      if (m < p) vals[mem[m]] = X[m];
+  }
   return 0;
 }
 
@@ -1642,7 +1723,7 @@ __global__ void __launch_bounds__(kV3Block) k_v3_run(const V3Launch *__restrict_
 // (group, signal) blocks: k_v3_refine lists the located entries of every
 // residue class (linked lists through head, W entries in cf, and next, M
 // entries in remi) and the non-empty classes; k_v3_refine_solve solves
-// those, one thread each (nsub workgroups per signal, the later ones
+// those, kRefOff lanes each (nsub workgroups per signal, the later ones
 // mostly idle); k_v3_refine_write sorts and writes the
 // result.  Groups without refine, and signals with the reference's early
 // return, wrote their result in k_v3_run and are skipped.
@@ -1679,18 +1760,20 @@ __global__ void __launch_bounds__(kV3Block) k_v3_refine_solve(const V3Launch *__
                                                               int nsub)
 {
   const int2 bb = blocks[blockIdx.x / (unsigned)nsub];
-  const int jj = (int)(blockIdx.x % (unsigned)nsub) * blockDim.x + threadIdx.x;
+  const int gid = (int)(blockIdx.x % (unsigned)nsub) * blockDim.x + threadIdx.x;
+  const int jj = gid / kRefOff, lane = gid % kRefOff;  // a group of kRefOff lanes a class
   const V3Launch &a = groups[bb.x];
   const int s = bb.y;
   if (!a.refine || (a.status[s] & 1)) return;  // no barriers here
-  // one thread per non-empty class (listed by k_v3_refine): every lane of
-  // a wave has a bucket to solve
+  // one group per non-empty class (listed by k_v3_refine); a group returns
+  // or continues as a whole, so its shuffles see all its lanes
   const int *nz = reinterpret_cast<const int *>(a.cv + (int64_t)s * a.Fcap);
   if (jj >= nz[0]) return;
   const int j = nz[1 + jj];
   const int g0 = a.params[(int64_t)s * 6 + 4];
-  if (refine_bucket(a, s, g0, j, a.cf + (int64_t)s * a.Fcap, a.remi + (int64_t)s * a.Rcap * 3,
-                    a.map_keys + (int64_t)s * a.Mcap, a.map_vals + (int64_t)s * a.Mcap))
+  if (refine_class(a, s, g0, j, a.cf + (int64_t)s * a.Fcap, a.remi + (int64_t)s * a.Rcap * 3,
+                   a.map_keys + (int64_t)s * a.Mcap, a.map_vals + (int64_t)s * a.Mcap, lane) &&
+      lane == 0)
     atomicOr(&a.status[s], 8);
 }
 
@@ -1760,7 +1843,7 @@ int v3_multi(const V3Launch *h_groups, int G, const V3Launch *d_groups, const in
   for (int g = 0; g < G; g++)
     if (h_groups[g].refine) {
       refine = true;
-      nsub = std::max(nsub, (h_groups[g].W + kV3Block - 1) / kV3Block);
+      nsub = std::max(nsub, (h_groups[g].W * kRefOff + kV3Block - 1) / kV3Block);
     }
   const int nref = refine ? kRefOff : 0;
   allow_v3_lds(reinterpret_cast<const void *>(k_v3_run), lds);
