@@ -991,6 +991,7 @@ static int exec_batch_v12(Plan *P, BatchIO &io, V12Pending *pd)
   a.fp32 = fp32;
   a.loc64 = loc64;
   a.ref_conj = P->ref_conj;
+  a.conj_sign = P->ref_conj ? 1.0 : -1.0;
   a.xs64 = need64 ? P->ws->xsamp.as<double2>() : nullptr;
   a.xs32 = fp32 ? P->ws->xsamp32.as<float2>() : nullptr;
   a.xs = xs;

@@ -335,6 +335,7 @@ struct V12Launch {
   bool fp32;       // estimation loops (and v2 location loops) in FP32
   bool loc64;      // v1 under fp32: location loops stay FP64
   bool ref_conj;
+  double conj_sign;  // 1.0 with ref_conj, else -1.0 (the FP32 form's estimate sign)
   double2 *xs64;   // S x xs buckets of the FP64 loops
   float2 *xs32;    // S x xs buckets of the FP32 loops
   int64_t xs;

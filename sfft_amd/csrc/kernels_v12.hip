@@ -749,6 +749,20 @@ __device__ __forceinline__ void divide_by_filter(double2 U, double2 G, double de
   vi = conj ? (U.x * G.y - U.y * G.x) / den : (U.y * G.x - U.x * G.y) / den;
 }
 
+// FP32 form: bucket u times g = G / |G|^2, one complex multiply instead of
+// two divisions, each part one product and one fused multiply-add.  sign
+// (V12Launch::conj_sign) is 1.0 for the reference's conjugated estimate and
+// -1.0 for the other, whose imaginary part is the exact negation; it is a
+// launch argument rather than a select on ref_conj, so the negation stays
+// one multiplication.  Every FP32 estimation path calls this, so they
+// agree bit for bit.
+__device__ __forceinline__ void mul_by_inverse(double ux, double uy, double2 g, double sign,
+                                               double &vr, double &vi)
+{
+  vr = fma(ux, g.x, uy * g.y);
+  vi = sign * fma(ux, g.y, -(uy * g.x));
+}
+
 // Bucket and filter value of frequency h in loop l, computed directly.
 __device__ __forceinline__ void hash_direct(const V12Launch &a, int l, unsigned ai,
                                             unsigned h, int &hashed, double2 &G)
@@ -846,6 +860,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
     const double2 *xs64 = a.xs64 + (int64_t)s * a.xs;
     const float2 *xs32 = FP32 ? a.xs32 + (int64_t)s * a.xs : nullptr;
     const bool conj = a.ref_conj;
+    const double sign = a.conj_sign;
     for (int e = threadIdx.x; e < kTI * kTJ; e += kBlock) {
       // re-read the per-loop scalars from LDS every hit instead of holding
       // 3 x LMAX of them in registers across the loop (occupancy)
@@ -870,11 +885,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
           double2 U;
           if (FP32) {
             const float2 u = xs32[s_off[l] + hashed];
-            U = make_double2((double)u.x, (double)u.y);
-            // g = G / |G|^2: one complex multiply instead of two divisions
-            vr[l] = U.x * g.x + U.y * g.y;
-            const double q = U.x * g.y - U.y * g.x;
-            vi[l] = conj ? q : -q;
+            mul_by_inverse((double)u.x, (double)u.y, g, sign, vr[l], vi[l]);
           } else {
             U = xs64[s_off[l] + hashed];
             const double den = t_den[l * kTI + il];
@@ -940,9 +951,7 @@ __global__ void __launch_bounds__(kBlock) k_estimate_comb(V12Launch a, int tiles
           const double2 U = load_bucket(a, FP32, isloc, off + hashed);
           const double2 g = t_g[l * kTI + il];
           if (FP32) {
-            // g = G / |G|^2: one complex multiply instead of two divisions
-            vr[l] = U.x * g.x + U.y * g.y;
-            vi[l] = a.ref_conj ? U.x * g.y - U.y * g.x : U.y * g.x - U.x * g.y;
+            mul_by_inverse(U.x, U.y, g, a.conj_sign, vr[l], vi[l]);
           } else {
             divide_by_filter(U, g, t_den[l * kTI + il], a.ref_conj, vr[l], vi[l]);
           }
@@ -1011,16 +1020,51 @@ __global__ void __launch_bounds__(kBlock) k_bucket_fft_rows(V12Launch a, int l0,
 // batches, 4 when a batch gives too few tiles to fill the chip.
 constexpr int kRI = 16, kRIsmall = 4;
 
+// Median value of one residue (table column il) at alias row jm: the L
+// bucket reads of the lane, each at byte offset t_base + (t_q + jm mod
+// rows) * sizeof(bucket) of the signal's bucket array xs (the alias-row
+// layout needs the bucket FFTs in LDS, so a signal's buckets span a few MB
+// and 32-bit offsets hold them).
+template <int LMAX, bool FULL, bool FP32, int RI>
+__device__ __forceinline__ double2 rows_median(const char *xs, const unsigned *t_base,
+                                               const int *t_q, const double2 *t_g,
+                                               const double *t_den, int il, int jm, int rows,
+                                               int L, bool conj, double sign)
+{
+  constexpr unsigned kElt = FP32 ? sizeof(float2) : sizeof(double2);
+  double vr[LMAX], vi[LMAX];
+#pragma unroll
+  for (int l = 0; l < LMAX; l++) {
+    if (FULL || l < L) {
+      const unsigned off =
+          t_base[l * RI + il] + ((unsigned)(t_q[l * RI + il] + jm) & (unsigned)(rows - 1)) * kElt;
+      const double2 g = t_g[l * RI + il];
+      if constexpr (FP32) {
+        const float2 u = *reinterpret_cast<const float2 *>(xs + off);
+        mul_by_inverse((double)u.x, (double)u.y, g, sign, vr[l], vi[l]);
+      } else {
+        // the reference's operation order (cf12 383-398), as k_estimate_comb
+        divide_by_filter(*reinterpret_cast<const double2 *>(xs + off), g, t_den[l * RI + il],
+                         conj, vr[l], vi[l]);
+      }
+    } else {
+      vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
+      vi[l] = vr[l];
+    }
+  }
+  return lower_median<LMAX>(vr, vi, FULL ? LMAX : L);
+}
+
 // FULL: L == LMAX, so the loop over loops has no branch and the compiler
 // issues all bucket loads of a frequency before the first wait.
 template <int LMAX, bool FULL, bool FP32, int RI>
 __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles_i, int tiles_j)
 {
-  __shared__ int t_rb[RI * LMAX];       // r * rows of hashed0
+  using T2 = typename std::conditional<FP32, float2, double2>::type;
+  __shared__ unsigned t_base[RI * LMAX];  // byte offset of loop l's row r: (off_l + r*rows) * elt
   __shared__ int t_q[RI * LMAX];        // q0 * u^-1 mod rows
   __shared__ double2 t_g[RI * LMAX];    // FP32 form: G / |G|^2; FP64: G
   __shared__ double t_den[FP32 ? 1 : RI * LMAX];  // FP64: |G|^2
-  __shared__ int s_off[LMAX];
   __shared__ double2 stage[64 * (RI + 1)];  // [row][residue], padded
   int s;
   int64_t c;
@@ -1032,7 +1076,6 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
   const int L = a.L;
   const int stride = 2 * L + max(a.comb_loops, 1);
   const int *approved = a.approved + (int64_t)s * a.comb_loops * a.num;
-  if (threadIdx.x < L) s_off[threadIdx.x] = (int)loop_offset(threadIdx.x, a.Lloc, a.B_loc, a.B_est);
   for (int e = threadIdx.x; e < RI * L; e += blockDim.x) {
     const int l = e / RI, il = e % RI;
     const int i = ti * RI + il;
@@ -1045,7 +1088,9 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
       const int d = B / rows;
       const unsigned uinv = inv_odd(ai) & (unsigned)(rows - 1);
       const double den = G.x * G.x + G.y * G.y;
-      t_rb[l * RI + il] = (hashed % d) * rows;
+      t_base[l * RI + il] =
+          (unsigned)(loop_offset(l, a.Lloc, a.B_loc, a.B_est) + (hashed % d) * rows) *
+          (unsigned)sizeof(T2);
       t_q[l * RI + il] = (int)(((unsigned)(hashed / d) * uinv) & (unsigned)(rows - 1));
       if (FP32) {
         t_g[l * RI + il] = make_double2(G.x / den, G.y / den);
@@ -1056,8 +1101,10 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
     }
   }
   __syncthreads();
-  const float2 *xs32 = FP32 ? a.xs32 + (int64_t)s * a.xs : nullptr;
-  const double2 *xs64 = FP32 ? nullptr : a.xs64 + (int64_t)s * a.xs;
+  const T2 *xsig;
+  if constexpr (FP32) xsig = a.xs32 + (int64_t)s * a.xs;
+  else xsig = a.xs64 + (int64_t)s * a.xs;
+  const char *xs = reinterpret_cast<const char *>(xsig);
   const bool conj = a.ref_conj;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int j = tj * 64 + lane;             // one alias row per lane
@@ -1066,28 +1113,9 @@ __global__ void __launch_bounds__(kBlock) k_estimate_rows(V12Launch a, int tiles
     asm volatile("" ::: "memory");
     const int il = wid * (RI / 4) + q;     // one residue per wavefront
     if (ti * RI + il >= nc) break;         // wave-uniform; no barrier inside
-    double vr[LMAX], vi[LMAX];
-#pragma unroll
-    for (int l = 0; l < LMAX; l++) {
-      if (FULL || l < L) {
-        const int pos = t_rb[l * RI + il] + ((t_q[l * RI + il] + jm) & (rows - 1));
-        const double2 g = t_g[l * RI + il];
-        if (FP32) {
-          const float2 u = xs32[s_off[l] + pos];
-          const double ux = (double)u.x, uy = (double)u.y;
-          vr[l] = ux * g.x + uy * g.y;
-          const double qq = ux * g.y - uy * g.x;
-          vi[l] = conj ? qq : -qq;
-        } else {
-          // the reference's operation order (cf12 383-398), as k_estimate_comb
-          divide_by_filter(xs64[s_off[l] + pos], g, t_den[l * RI + il], conj, vr[l], vi[l]);
-        }
-      } else {
-        vr[l] = __longlong_as_double(0x7ff0000000000000ll);  // +inf pads
-        vi[l] = vr[l];
-      }
-    }
-    stage[lane * (RI + 1) + il] = lower_median<LMAX>(vr, vi, FULL ? LMAX : L);
+    stage[lane * (RI + 1) + il] =
+        rows_median<LMAX, FULL, FP32, RI>(xs, t_base, t_q, t_g, t_den, il, jm, rows, L, conj,
+                                          a.conj_sign);
   }
   __syncthreads();
   // row by row: RI consecutive output slots per row

@@ -253,6 +253,32 @@ def test_schedules_give_identical_results(torch_cuda, version, n, k, snr, precis
         assert np.array_equal(va, res[0][1])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refine"])
+def test_unconjugated_values(torch_cuda, precision):
+    """With the reference's conjugation undone, every schedule (the alias-row
+    estimation of schedule 1 and the natural-order one of schedule 2) gives
+    the same bits, the located set and the real parts are the conjugated
+    form's, and each tone's value is the conjugate of the default one up to
+    the choice between the two middle estimates of its imaginary part."""
+    n, k = 1 << 20, 64
+    print(f"case v2 n={n} k={k} {precision} unconjugated", flush=True)
+    x, f, v = signal_np(n, k, seed=6100, snr_db=30.0)
+    base = _gpu_plan(2, n, k, precision).execute_sparse_host(x)
+    res = []
+    for mode in (1, 2):
+        plan = _gpu_plan(2, n, k, precision)
+        plan.set_reference_conjugate(False)
+        plan.set_schedule(mode, 0)
+        res.append(plan.execute_sparse_host(x))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    fr, va = res[0]
+    assert np.array_equal(fr, base[0])
+    assert np.array_equal(va.real, base[1].real)
+    tone = np.isin(fr, f)
+    assert tone.sum() == k
+    assert np.abs(va[tone] - np.conj(base[1][tone])).max() < 0.05
+
+
 def test_autotune_keeps_draws_and_results(torch_cuda):
     n, k = 1 << 20, 64
     x, f, v = signal_np(n, k, seed=77, snr_db=30.0)
